@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""H-matrix-vector product throughput on one MI355X.
+
+Workload: kernel 1/|x-y| on n = 262144 points uniform on the unit sphere
+(diagonal = mean of the 8 nearest-neighbour entries), binary space
+partitioning with leaf size 64, standard admissibility eta = 2, ACA + SVD
+truncation to eps = 1e-6, x uniform in [-1, 1], nrhs = 1.
+
+One step = one H-MVM y += 2 A x over the whole matrix.  The mixed-precision
+plan (per-column FP32/BF16/FP8-E4M3 chosen from singular values against eps,
+dense leaves FP32, FP32 accumulation inside panels) is the measured path;
+the plain FP64 plan is timed beside it, both with factors resident in HBM.
+The relative error of the mixed-precision result is taken against the FP64
+GPU result and the CPU oracle.  The CPU baseline is the oracle's restatement
+of the reference's TBB mul_vec_chunk on the host cores.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="timed H-MVMs per precision")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the H-MVM path here is single-GPU "
+              "(row-panel sharding over several GPUs is described in DESIGN.md, not built).",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from hlr_amd import _native as N
+    from hlr_amd.hmatrix import INVR_SPHERE, HMatrix, random_vector
+    from hlr_amd.plan import Plan
+
+    dev = torch.device("cuda:0")
+    prob = INVR_SPHERE.with_(n=args.n, eps=args.eps)
+    t0 = time.time()
+    hm = HMatrix(prob)
+    st = hm.stats()
+    log(f"matrix n={hm.n} leaves={st['leaves']} dense={st['dense']} lowrank={st['lowrank']} "
+        f"kmax={st['kmax']} fp64 bytes={(st['dense_bytes_fp64'] + st['lowrank_bytes_fp64']) / 1e9:.2f} GB "
+        f"built in {time.time() - t0:.1f} s")
+
+    x_h = random_vector(hm.n, 2)
+    x = torch.as_tensor(x_h, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def time_plan(plan, steps, warmup):
+        y = torch.zeros(hm.n, dtype=torch.float64, device=dev)
+        for _ in range(warmup):
+            plan.apply(2.0, N.OP_NORMAL, x, y)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            plan.apply(2.0, N.OP_NORMAL, x, y)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        # per-phase durations (HIP events on the launch stream)
+        phases = np.array([plan.apply_timed(2.0, N.OP_NORMAL, x, y) for _ in range(max(5, steps // 10))])
+        return ms, phases.mean(axis=0)
+
+    def result(plan):
+        y = torch.zeros(hm.n, dtype=torch.float64, device=dev)
+        plan.apply(1.0, N.OP_NORMAL, x, y)
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    # ---- plain FP64 plan ----
+    t0 = time.time()
+    p64 = Plan.from_hmatrix(hm, precision="fp64")
+    i64 = p64.info()
+    log(f"fp64 plan: {i64['factor_bytes'] / 1e9:.2f} GB in {time.time() - t0:.1f} s")
+    ms64, ph64 = time_plan(p64, args.steps, args.warmup)
+    y64 = result(p64)
+    log(f"fp64: {ms64:.3f} ms/MVM  phases {ph64.round(4).tolist()} ms")
+    p64.close()
+    torch.cuda.empty_cache()
+
+    # ---- mixed-precision plan ----
+    t0 = time.time()
+    pmp = Plan.from_hmatrix(hm, precision="valr", eps=args.eps)
+    imp = pmp.info()
+    log(f"valr plan: {imp['factor_bytes'] / 1e9:.2f} GB cols/format {imp['cols_by_format']} "
+        f"in {time.time() - t0:.1f} s")
+    msmp, phmp = time_plan(pmp, args.steps, args.warmup)
+    ymp = result(pmp)
+    log(f"valr: {msmp:.3f} ms/MVM  phases {phmp.round(4).tolist()} ms")
+    pmp.close()
+
+    err_gpu = float(np.linalg.norm(ymp - y64) / np.linalg.norm(y64))
+    err_oracle = None
+    err64_oracle = None
+    cpu = None
+    if not args.no_oracle or not args.no_cpu:
+        import oracle
+        if not args.no_oracle:
+            t0 = time.time()
+            yr = np.zeros(hm.n)
+            oracle.mul_vec(hm.blocks, hm.arena, 1.0, N.OP_NORMAL, x_h, yr)
+            err_oracle = float(np.linalg.norm(ymp - yr) / np.linalg.norm(yr))
+            err64_oracle = float(np.linalg.norm(y64 - yr) / np.linalg.norm(yr))
+            log(f"oracle: {time.time() - t0:.1f} s; err(valr)={err_oracle:.3e} err(fp64)={err64_oracle:.3e}")
+        if not args.no_cpu:
+            cores = len(os.sched_getaffinity(0))
+            cores = min(cores, 16)
+            yc = np.zeros(hm.n)
+            t1 = oracle.time_chunk(hm.blocks, hm.arena, x_h, yc, cores, 1)
+            nmvm = max(1, int(args.cpu_seconds / max(t1, 1e-6)))
+            tc = oracle.time_chunk(hm.blocks, hm.arena, x_h, yc, cores, nmvm)
+            cpu = {"value": nmvm / tc, "unit": "H-MVM/s", "cores": cores, "kind": "port",
+                   "sample": f"{nmvm} FP64 H-MVMs of the same matrix with the oracle's restatement of "
+                             f"tbb::mul_vec_chunk (64-entry locked chunks), {cores} threads, {tc:.1f} s"}
+            log(f"cpu baseline: {cpu['value']:.3f} MVM/s on {cores} threads")
+
+    # ---- roofline of the dominant kernel: the row-panel pass (k_ufwd) ----
+    n = hm.n
+    vec_bytes = 8.0 * (n + n)                   # read x, read+write y once per pass
+    r_bytes = imp["bytes_rpass"] + vec_bytes
+    t_r = phmp[2] * 1e-3
+    achieved = r_bytes / t_r / 1e9
+    mvm_bytes = imp["datasize"]
+    gbs_mp = mvm_bytes / (msmp * 1e-3) / 1e9
+    gbs_64 = i64["datasize"] / (ms64 * 1e-3) / 1e9
+
+    out = {
+        "metric": "H-MVM/s + HBM GB/s (% peak), mixed-prec vs FP64 GPU at rel.err<=eps, n=262k",
+        "value": 1000.0 / msmp,
+        "unit": "H-MVM/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": msmp,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-acc mixed fp32/bf16/fp8-e4m3 storage",
+        "data": "synthetic (seeded points, generated H-matrix)",
+        "config": {"workload": f"1/r on unit sphere, n={n}, leaf 64, eta 2, ACA eps={args.eps:g}, nrhs 1",
+                   "precision": "valr", "parallelism": "single GPU"},
+        "fp64_mvm_per_s": 1000.0 / ms64,
+        "fp64_ms_per_step": ms64,
+        "speedup_vs_fp64": ms64 / msmp,
+        "rel_err_vs_fp64_gpu": err_gpu,
+        "rel_err_vs_oracle": err_oracle,
+        "fp64_rel_err_vs_oracle": err64_oracle,
+        "hbm_gbs_mixed": gbs_mp,
+        "hbm_gbs_fp64": gbs_64,
+        "hbm_frac_mixed": gbs_mp / HBM_PEAK_GBS,
+        "hbm_frac_fp64": gbs_64 / HBM_PEAK_GBS,
+        "bytes_per_mvm_mixed": mvm_bytes,
+        "bytes_per_mvm_fp64": i64["datasize"],
+        "phases_ms_mixed": phmp.tolist(),
+        "phases_ms_fp64": ph64.tolist(),
+        "roofline": {"bound": "hbm", "kernel": "k_ufwd (row-panel pass, mixed precision)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,117 @@
+// kernels.hpp -- CDNA4 (gfx950) kernels of the H-/TLR-matrix-vector product.
+//
+// One wavefront = 64 lanes.  Factor data is read with one 16-byte load per
+// lane (W = 16 / sizeof(element) values), decoded to the accumulation type,
+// and never re-read: the forward product touches every stored byte once.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace hmvm {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// device view of a plan (passed by value)
+struct DevPlan {
+    const RowPanel* rp;
+    const RChunk* rchunks;
+    const ColPanel* cp;
+    const VItem* vitems;
+    const int32_t* colsrc;
+    const float* zscale;      // null: all 1
+    const int32_t* rdest;
+    const int32_t* vdest;
+    const int32_t* vsrc;
+    const float* vscale;      // null: all 1
+    const double* tmul;
+    const int32_t* pt_off;
+    const int32_t* pt_cnt;
+    const int32_t* qt_off;
+    const int32_t* qt_cnt;
+    const int32_t* qd_off;
+    const int32_t* qd_cnt;
+    const void* arena[NFMT];
+    int64_t nrp, ncp, nvitems, nrchunks, ntent, psize, qsize, nrows, ncols;
+};
+
+// ---------------------------------------------------------------------------
+// 16-byte loads of W elements, decoded
+// ---------------------------------------------------------------------------
+
+template <typename T> struct Elem;
+
+template <> struct Elem<double> {
+    static constexpr int W = 2;
+    using Acc = double;
+    __device__ static inline void load(const double* p, double* out) {
+        const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p));
+        out[0] = v.x; out[1] = v.y;
+    }
+};
+
+template <> struct Elem<float> {
+    static constexpr int W = 4;
+    using Acc = float;
+    __device__ static inline void load(const float* p, float* out) {
+        const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
+        out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+    }
+};
+
+template <> struct Elem<uint16_t> {   // BF16
+    static constexpr int W = 8;
+    using Acc = float;
+    __device__ static inline void load(const uint16_t* p, float* out) {
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            out[2 * i] = __uint_as_float(w[i] << 16);
+            out[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+        }
+    }
+};
+
+template <> struct Elem<uint8_t> {    // FP8 E4M3 (OCP)
+    static constexpr int W = 16;
+    using Acc = float;
+    __device__ static inline void load(const uint8_t* p, float* out) {
+        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[i]), false);
+            const f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(int(w[i]), true);
+            out[4 * i] = lo.x; out[4 * i + 1] = lo.y; out[4 * i + 2] = hi.x; out[4 * i + 3] = hi.y;
+        }
+    }
+};
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// sum v[c] over the 64 lanes for c = 0..63; lane L returns the sum for c = L
+template <typename A>
+__device__ inline A butterfly64(A (&v)[64], int lane) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        const bool up = (lane & s) != 0;
+#pragma unroll
+        for (int j = 0; j < s; ++j) {
+            const A send = up ? v[j] : v[j + s];
+            const A keep = up ? v[j + s] : v[j];
+            v[j] = keep + __shfl_xor(send, s, 64);
+        }
+    }
+    return v[0];
+}
+
+} // namespace hmvm

@@ -1,0 +1,644 @@
+// plan_build.cpp -- validation, precision choice, panel layout and packing.
+//
+// Host-only code.  Builds the device layout described in plan.hpp from the
+// flattened leaves.  The precision rule restates the reference's VALR
+// chooser: per-column tolerance u_l = delta / sigma_l from
+// valr::get_tolerances (compress/valr.hh:402-512), mantissa bits
+// m_l = max(1, ceil(-log2 u_l)) from afl::eps_to_rate (compress/afl.hh:81-82),
+// and per-column scaling as in afl::compress_lr (compress/afl.hh:818-885);
+// here m_l selects the narrowest hardware format with at least m_l mantissa
+// bits (FP8-E4M3: 3, BF16: 7, FP32: 23, FP64: 52) instead of a bitstream.
+
+#include "plan_host.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <omp.h>
+
+namespace hmvm {
+
+// ---------------------------------------------------------------------------
+// number formats
+// ---------------------------------------------------------------------------
+
+uint8_t encode_e4m3(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    const uint8_t sign = uint8_t((u >> 24) & 0x80u);
+    u &= 0x7fffffffu;
+    if (u > 0x7f800000u) return 0x7f;                  // NaN
+    const float a = std::fabs(v);
+    if (!(a < 464.0f)) return sign | 0x7e;             // saturate at 448
+    if (a < 0.015625f) {                               // below 2^-6: multiples of 2^-9
+        const float q = std::nearbyint(a * 512.0f);    // exact scaling, RNE
+        return sign | uint8_t(q);                      // q == 8 encodes 2^-6 (e=1, m=0)
+    }
+    int e = int(u >> 23) - 127;
+    const uint32_t m23 = u & 0x7fffffu;
+    uint32_t r = m23 >> 20;
+    const uint32_t rem = m23 & 0xfffffu;
+    if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) ++r;
+    if (r == 8) { r = 0; ++e; }
+    uint32_t code = (uint32_t(e + 7) << 3) | r;
+    if (code > 0x7eu) code = 0x7eu;
+    return sign | uint8_t(code);
+}
+
+float decode_e4m3(uint8_t c) {
+    const int s = c >> 7, e = (c >> 3) & 0xf, m = c & 7;
+    if (e == 0xf && m == 7) return std::nanf("");
+    float v = (e == 0) ? std::ldexp(float(m), -9) : std::ldexp(1.0f + float(m) / 8.0f, e - 7);
+    return s ? -v : v;
+}
+
+uint16_t encode_bf16(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+float decode_bf16(uint16_t c) {
+    const uint32_t u = uint32_t(c) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+namespace {
+
+int eps_to_rate(double eps) {
+    const double r = std::ceil(-std::log2(eps));
+    return int(r < 1.0 ? 1.0 : r);
+}
+
+int format_for_rate(int rate) {
+    if (rate <= 3) return FP8;
+    if (rate <= 7) return BF16;
+    if (rate <= 23) return F32;
+    return F64;
+}
+
+void valr_tolerances(int64_t k, const double* sv, double eps, double* out) {
+    double tol = eps * sv[0];
+    double sum = 0;
+    for (int64_t i = 0; i < k; ++i) sum += 1.0 / sv[i];
+    const double p = 1.0 / sum, q = tol / sum;
+    double t = p * p / 4 - q;
+    if (t < 0) t = 0;
+    const double d1 = std::fabs(p / 2 + std::sqrt(t)), d2 = std::fabs(p / 2 - std::sqrt(t));
+    tol = std::min(d1, d2);
+    for (int64_t l = 0; l < k; ++l) out[l] = tol / sv[l];
+}
+
+// split [a,b) into pieces of at most kPanel, as even as possible
+void split_interval(int64_t a, int64_t b, std::vector<std::pair<int64_t, int32_t>>& out) {
+    const int64_t len = b - a;
+    if (len <= 0) return;
+    const int64_t np = (len + kPanel - 1) / kPanel;
+    const int64_t base = len / np, extra = len % np;
+    int64_t s = a;
+    for (int64_t i = 0; i < np; ++i) {
+        const int64_t l = base + (i < extra ? 1 : 0);
+        out.push_back({s, int32_t(l)});
+        s += l;
+    }
+}
+
+int64_t round_up(int64_t v, int64_t w) { return (v + w - 1) / w * w; }
+
+// power-of-two scale so that max |v| / s <= 448
+float fp8_scale(double maxabs) {
+    if (!(maxabs > 0) || !std::isfinite(maxabs)) return 1.0f;
+    int e = int(std::ceil(std::log2(maxabs / 448.0)));
+    float s = std::ldexp(1.0f, e);
+    while (maxabs / s > 448.0) s *= 2;     // guard against log2 rounding
+    return s;
+}
+
+struct Seg {              // a block's contribution to one panel
+    int64_t b;            // block
+    int32_t pi;           // panel index inside the block's panel range
+};
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// layout
+// ---------------------------------------------------------------------------
+
+int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t ncols, const double* arena,
+                 int64_t arena_len, const hmvm_plan_opts& opts, HostLayout& L) {
+    if (nb < 0 || nrows < 0 || ncols < 0 || (nb > 0 && (!blocks || !arena))) return HMVM_ERR_INVALID_ARG;
+    if (opts.precision < HMVM_PREC_FP64 || opts.precision > HMVM_PREC_VALR) return HMVM_ERR_INVALID_ARG;
+    if (opts.precision == HMVM_PREC_VALR && !(opts.eps > 0)) return HMVM_ERR_INVALID_ARG;
+    if (nrows > INT32_MAX || ncols > INT32_MAX) return HMVM_ERR_UNSUPPORTED;
+    L = HostLayout();
+    L.nrows = nrows;
+    L.ncols = ncols;
+    L.precision = opts.precision;
+    L.eps = opts.eps;
+    L.dense_fmt = (opts.precision == HMVM_PREC_FP64) ? F64
+                : (opts.precision == HMVM_PREC_FP32) ? F32
+                : (opts.dense_prec == HMVM_PREC_FP64 ? F64 : F32);
+    L.blocks.assign(blocks, blocks + nb);
+
+    // ---- validate leaves (cf. HLR_ASSERT in apply_add, check_data()) ----
+    for (int64_t i = 0; i < nb; ++i) {
+        const hmvm_block& b = blocks[i];
+        if (b.kind < HMVM_BLOCK_DENSE || b.kind > HMVM_BLOCK_LOWRANK_SV) return HMVM_ERR_BAD_BLOCK;
+        if (b.nrows < 0 || b.ncols < 0 || b.row_ofs < 0 || b.col_ofs < 0) return HMVM_ERR_BAD_BLOCK;
+        if (b.row_ofs + b.nrows > nrows || b.col_ofs + b.ncols > ncols) return HMVM_ERR_BAD_BLOCK;
+        const int64_t m = b.nrows, n = b.ncols;
+        if (b.kind == HMVM_BLOCK_DENSE) {
+            if (m * n > 0 && (b.off_a < 0 || b.off_a + m * n > arena_len)) return HMVM_ERR_BAD_BLOCK;
+        } else {
+            const int64_t k = b.rank;
+            if (k < 0 || k > std::min(m, n)) return HMVM_ERR_BAD_BLOCK;
+            if (k > 0) {
+                if (b.off_a < 0 || b.off_a + m * k > arena_len) return HMVM_ERR_BAD_BLOCK;
+                if (b.off_b < 0 || b.off_b + n * k > arena_len) return HMVM_ERR_BAD_BLOCK;
+            }
+            if (b.kind == HMVM_BLOCK_LOWRANK_SV && k > 0 && (b.off_s < 0 || b.off_s + k > arena_len))
+                return HMVM_ERR_BAD_BLOCK;
+            if (b.off_s >= 0 && k > 0 && b.off_s + k > arena_len) return HMVM_ERR_BAD_BLOCK;
+        }
+    }
+    if (opts.check_finite) {
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+        for (int64_t i = 0; i < nb; ++i) {
+            const hmvm_block& b = blocks[i];
+            auto scan = [&](int64_t off, int64_t len) {
+                for (int64_t t = 0; t < len; ++t)
+                    if (!std::isfinite(arena[off + t])) return 1;
+                return 0;
+            };
+            if (b.kind == HMVM_BLOCK_DENSE) bad |= scan(b.off_a, int64_t(b.nrows) * b.ncols);
+            else if (b.rank > 0) {
+                bad |= scan(b.off_a, int64_t(b.nrows) * b.rank);
+                bad |= scan(b.off_b, int64_t(b.ncols) * b.rank);
+                if (b.kind == HMVM_BLOCK_LOWRANK_SV) bad |= scan(b.off_s, b.rank);
+            }
+        }
+        if (bad) return HMVM_ERR_NONFINITE;
+    }
+
+    // ---- panels ----
+    std::vector<int64_t> rb = {0, nrows}, cb = {0, ncols};
+    for (int64_t i = 0; i < nb; ++i) {
+        rb.push_back(blocks[i].row_ofs); rb.push_back(blocks[i].row_ofs + blocks[i].nrows);
+        cb.push_back(blocks[i].col_ofs); cb.push_back(blocks[i].col_ofs + blocks[i].ncols);
+    }
+    std::sort(rb.begin(), rb.end()); rb.erase(std::unique(rb.begin(), rb.end()), rb.end());
+    std::sort(cb.begin(), cb.end()); cb.erase(std::unique(cb.begin(), cb.end()), cb.end());
+    std::vector<std::pair<int64_t, int32_t>> rpan, cpan;
+    for (size_t i = 0; i + 1 < rb.size(); ++i) split_interval(rb[i], rb[i + 1], rpan);
+    for (size_t i = 0; i + 1 < cb.size(); ++i) split_interval(cb[i], cb[i + 1], cpan);
+    std::vector<int64_t> rstart(rpan.size() + 1), cstart(cpan.size() + 1);
+    for (size_t i = 0; i < rpan.size(); ++i) rstart[i] = rpan[i].first;
+    rstart[rpan.size()] = nrows;
+    for (size_t i = 0; i < cpan.size(); ++i) cstart[i] = cpan[i].first;
+    cstart[cpan.size()] = ncols;
+    auto find_panel = [](const std::vector<int64_t>& st, int64_t v) {
+        return int64_t(std::lower_bound(st.begin(), st.end(), v) - st.begin());
+    };
+    const int64_t NR = int64_t(rpan.size()), NC = int64_t(cpan.size());
+    std::vector<int64_t> bp0(nb), bp1(nb), bq0(nb), bq1(nb);
+    for (int64_t i = 0; i < nb; ++i) {
+        const hmvm_block& b = blocks[i];
+        bp0[i] = find_panel(rstart, b.row_ofs);
+        bp1[i] = find_panel(rstart, b.row_ofs + b.nrows);
+        bq0[i] = find_panel(cstart, b.col_ofs);
+        bq1[i] = find_panel(cstart, b.col_ofs + b.ncols);
+        if (b.nrows == 0 || b.ncols == 0) { bp1[i] = bp0[i]; bq1[i] = bq0[i]; }
+    }
+
+    // ---- t entries and their formats ----
+    L.blk_t0.assign(nb, -1);
+    int64_t ne = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+        if (blocks[i].kind == HMVM_BLOCK_DENSE) { ++L.ndense; continue; }
+        ++L.nlowrank;
+        L.blk_t0[i] = ne;
+        ne += blocks[i].rank;
+    }
+    if (ne > INT32_MAX / 2) return HMVM_ERR_UNSUPPORTED;
+    L.ntent = ne;
+    L.tfmt.assign(ne, int8_t(F64));
+    L.tmul.assign(ne, 1.0);
+    {
+        std::vector<double> tol;
+        for (int64_t i = 0; i < nb; ++i) {
+            const hmvm_block& b = blocks[i];
+            if (b.kind == HMVM_BLOCK_DENSE || b.rank == 0) continue;
+            const int64_t e0 = L.blk_t0[i];
+            if (b.kind == HMVM_BLOCK_LOWRANK_SV)
+                for (int64_t l = 0; l < b.rank; ++l) L.tmul[e0 + l] = arena[b.off_s + l];
+            int f = F64;
+            if (opts.precision == HMVM_PREC_FP32) f = F32;
+            for (int64_t l = 0; l < b.rank; ++l) L.tfmt[e0 + l] = int8_t(f);
+            if (opts.precision == HMVM_PREC_VALR && b.off_s >= 0) {
+                tol.resize(b.rank);
+                valr_tolerances(b.rank, arena + b.off_s, opts.eps, tol.data());
+                for (int64_t l = 0; l < b.rank; ++l) L.tfmt[e0 + l] = int8_t(format_for_rate(eps_to_rate(tol[l])));
+            }
+            for (int64_t l = 0; l < b.rank; ++l) L.cols_by_format[L.tfmt[e0 + l]]++;
+        }
+    }
+
+    // ---- overlap check: per row panel, column intervals must be disjoint ----
+    std::vector<std::vector<Seg>> rseg(NR), cseg(NC);
+    for (int64_t i = 0; i < nb; ++i) {
+        for (int64_t p = bp0[i]; p < bp1[i]; ++p) rseg[p].push_back({i, int32_t(p - bp0[i])});
+        if (blocks[i].kind != HMVM_BLOCK_DENSE && blocks[i].rank > 0)
+            for (int64_t q = bq0[i]; q < bq1[i]; ++q) cseg[q].push_back({i, int32_t(q - bq0[i])});
+    }
+    {
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : bad)
+        for (int64_t p = 0; p < NR; ++p) {
+            std::vector<std::pair<int64_t, int64_t>> iv;
+            for (const Seg& s : rseg[p]) {
+                const hmvm_block& b = blocks[s.b];
+                if (b.ncols > 0) iv.push_back({b.col_ofs, b.col_ofs + b.ncols});
+            }
+            std::sort(iv.begin(), iv.end());
+            for (size_t t = 1; t < iv.size(); ++t)
+                if (iv[t].first < iv[t - 1].second) bad = 1;
+        }
+        if (bad) return HMVM_ERR_OVERLAP;
+    }
+
+    // ---- forward partial layout P (t-entry major) and transposed Q ----
+    L.pt_off.assign(ne, 0); L.pt_cnt.assign(ne, 0);
+    L.qt_off.assign(ne, 0); L.qt_cnt.assign(ne, 0);
+    {
+        int64_t pos = 0, qpos = 0;
+        for (int64_t i = 0; i < nb; ++i) {
+            if (L.blk_t0[i] < 0) continue;
+            for (int64_t l = 0; l < blocks[i].rank; ++l) {
+                const int64_t e = L.blk_t0[i] + l;
+                L.pt_off[e] = int32_t(pos); L.pt_cnt[e] = int32_t(bq1[i] - bq0[i]); pos += bq1[i] - bq0[i];
+                L.qt_off[e] = int32_t(qpos); L.qt_cnt[e] = int32_t(bp1[i] - bp0[i]); qpos += bp1[i] - bp0[i];
+            }
+        }
+        L.psize = pos;
+        // dense transposed partials: per global column, one per (dense block, row panel)
+        L.qd_cnt.assign(ncols, 0);
+        L.qd_off.assign(ncols, 0);
+        for (int64_t i = 0; i < nb; ++i) {
+            if (blocks[i].kind != HMVM_BLOCK_DENSE) continue;
+            for (int64_t j = 0; j < blocks[i].ncols; ++j) L.qd_cnt[blocks[i].col_ofs + j] += int32_t(bp1[i] - bp0[i]);
+        }
+        for (int64_t j = 0; j < ncols; ++j) { L.qd_off[j] = int32_t(qpos); qpos += L.qd_cnt[j]; }
+        L.qsize = qpos;
+        if (L.qsize > INT32_MAX / 2 || L.psize > INT32_MAX / 2) return HMVM_ERR_UNSUPPORTED;
+    }
+    // per (dense block, row panel, column) destination of its transposed
+    // partial, assigned in block order then panel order
+    std::vector<int64_t> dstart(nb, -1);    // index into ddest of the block's first entry
+    std::vector<int32_t> ddest;
+    {
+        std::vector<int32_t> cur(ncols, 0);
+        int64_t cnt = 0;
+        for (int64_t i = 0; i < nb; ++i)
+            if (blocks[i].kind == HMVM_BLOCK_DENSE) cnt += (bp1[i] - bp0[i]) * int64_t(blocks[i].ncols);
+        ddest.resize(cnt);
+        int64_t pos = 0;
+        for (int64_t i = 0; i < nb; ++i) {
+            if (blocks[i].kind != HMVM_BLOCK_DENSE) continue;
+            dstart[i] = pos;
+            for (int64_t p = bp0[i]; p < bp1[i]; ++p)
+                for (int64_t j = 0; j < blocks[i].ncols; ++j) {
+                    const int64_t g = blocks[i].col_ofs + j;
+                    ddest[pos++] = L.qd_off[g] + cur[g]++;
+                }
+        }
+    }
+
+    // ---- row panels: column counts per format, chunks ----
+    const int dfmt = L.dense_fmt;
+    L.rp.resize(NR);
+    std::vector<std::array<int64_t, NFMT>> rcols(NR);
+    for (int64_t p = 0; p < NR; ++p) {
+        std::array<int64_t, NFMT> c = {0, 0, 0, 0};
+        for (const Seg& s : rseg[p]) {
+            const hmvm_block& b = blocks[s.b];
+            if (b.kind == HMVM_BLOCK_DENSE) c[dfmt] += b.ncols;
+            else for (int64_t l = 0; l < b.rank; ++l) c[L.tfmt[L.blk_t0[s.b] + l]]++;
+        }
+        rcols[p] = c;
+    }
+    // chunk table and offsets
+    int64_t arena_pos[NFMT] = {0, 0, 0, 0};
+    int64_t colpos = 0;
+    std::vector<std::array<int64_t, NFMT>> rcol0(NR);        // first per-column index of format f in panel p
+    for (int64_t p = 0; p < NR; ++p) {
+        RowPanel& P = L.rp[p];
+        P.row0 = rpan[p].first;
+        P.m = rpan[p].second;
+        P.chunk0 = int64_t(L.rchunks.size());
+        for (int f = 0; f < NFMT; ++f) {
+            const int W = fmt_width(f);
+            const int64_t kp = round_up(rcols[p][f], W);
+            rcol0[p][f] = colpos;
+            const int64_t ng = kp / W;
+            for (int64_t g0 = 0; g0 < ng; g0 += kChunkGroups) {
+                RChunk c;
+                c.fmt = f;
+                c.ngroups = int32_t(std::min<int64_t>(kChunkGroups, ng - g0));
+                c.data_off = arena_pos[f] + g0 * P.m * W;
+                c.col0 = colpos + g0 * W;
+                c.panel = int32_t(p);
+                c.pad_ = 0;
+                L.rchunks.push_back(c);
+            }
+            arena_pos[f] += kp * P.m;
+            colpos += kp;
+        }
+        P.nchunks = int32_t(int64_t(L.rchunks.size()) - P.chunk0);
+    }
+    const int64_t nrcols = colpos;
+
+    // ---- column panels ----
+    L.cp.resize(NC);
+    std::vector<std::array<int64_t, NFMT>> ccols(NC);
+    for (int64_t q = 0; q < NC; ++q) {
+        std::array<int64_t, NFMT> c = {0, 0, 0, 0};
+        for (const Seg& s : cseg[q])
+            for (int64_t l = 0; l < blocks[s.b].rank; ++l) c[L.tfmt[L.blk_t0[s.b] + l]]++;
+        ccols[q] = c;
+    }
+    int64_t vpos = 0;
+    int64_t varena_pos[NFMT] = {arena_pos[0], arena_pos[1], arena_pos[2], arena_pos[3]};
+    for (int64_t q = 0; q < NC; ++q) {
+        ColPanel& C = L.cp[q];
+        C.col0 = cpan[q].first;
+        C.m = cpan[q].second;
+        C.pad_ = 0;
+        for (int f = 0; f < NFMT; ++f) {
+            const int W = fmt_width(f);
+            const int64_t kp = round_up(ccols[q][f], W);
+            C.kpad[f] = int32_t(kp);
+            C.off[f] = varena_pos[f];
+            C.vcol0[f] = vpos;
+            varena_pos[f] += kp * C.m;
+            vpos += kp;
+            const int64_t ng = kp / W;
+            // work items of <= 64 groups, as even as possible
+            if (ng > 0) {
+                const int64_t nit = (ng + 63) / 64;
+                const int64_t base = ng / nit, extra = ng % nit;
+                int64_t g = 0;
+                for (int64_t it = 0; it < nit; ++it) {
+                    const int64_t cnt = base + (it < extra ? 1 : 0);
+                    L.vitems.push_back({int32_t(q), int32_t(f), int32_t(g), int32_t(cnt)});
+                    g += cnt;
+                }
+            }
+        }
+        L.max_vitem_rows = std::max<int32_t>(L.max_vitem_rows, C.m);
+    }
+    const int64_t nvcols = vpos;
+    if (nrcols > INT32_MAX || nvcols > INT32_MAX) return HMVM_ERR_UNSUPPORTED;
+
+    // ---- allocate arenas and tables ----
+    L.a64.assign(varena_pos[F64], 0.0);
+    L.a32.assign(varena_pos[F32], 0.0f);
+    L.a16.assign(varena_pos[BF16], 0);
+    L.a8.assign(varena_pos[FP8], 0);
+    L.colsrc.assign(nrcols, kPadCol);
+    L.rdest.assign(nrcols, -1);
+    L.vdest.assign(nvcols, -1);
+    L.vsrc.assign(nvcols, -1);
+    const bool need_scale = (opts.precision != HMVM_PREC_FP64);
+    if (need_scale) {
+        L.zscale.assign(nrcols, 1.0f);
+        L.vscale.assign(nvcols, 1.0f);
+    }
+
+    // store one column of m values (source stride 1) in format f at group layout
+    auto store_col = [&](int f, int64_t base, int64_t m, int64_t W, int64_t c, const double* src,
+                         float* scale_out) {
+        // column c of a format group block starting at element `base` with
+        // groups of W columns, each m x W row-major
+        const int64_t g = c / W, w = c % W;
+        const int64_t o = base + g * m * W + w;
+        switch (f) {
+        case F64: for (int64_t i = 0; i < m; ++i) L.a64[o + i * W] = src[i]; break;
+        case F32: for (int64_t i = 0; i < m; ++i) L.a32[o + i * W] = float(src[i]); break;
+        case BF16: for (int64_t i = 0; i < m; ++i) L.a16[o + i * W] = encode_bf16(float(src[i])); break;
+        case FP8: {
+            double mx = 0;
+            for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
+            const float s = fp8_scale(mx);
+            for (int64_t i = 0; i < m; ++i) L.a8[o + i * W] = encode_e4m3(float(src[i] / double(s)));
+            if (scale_out) *scale_out = s;
+            break;
+        }
+        }
+    };
+
+    // ---- fill row panels (parallel over panels) ----
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t p = 0; p < NR; ++p) {
+        const RowPanel& P = L.rp[p];
+        const int64_t m = P.m;
+        int64_t cnt[NFMT] = {0, 0, 0, 0};
+        // element base of format f block of this panel = data_off of its first chunk
+        int64_t fbase[NFMT] = {-1, -1, -1, -1};
+        for (int64_t c = P.chunk0; c < P.chunk0 + P.nchunks; ++c)
+            if (fbase[L.rchunks[c].fmt] < 0) fbase[L.rchunks[c].fmt] = L.rchunks[c].data_off;
+        std::vector<double> tmp(m);
+        for (const Seg& s : rseg[p]) {
+            const hmvm_block& b = blocks[s.b];
+            const int64_t r0 = P.row0 - b.row_ofs;     // first row of the panel inside the block
+            if (b.kind == HMVM_BLOCK_DENSE) {
+                const int f = dfmt, W = fmt_width(f);
+                for (int64_t j = 0; j < b.ncols; ++j) {
+                    const int64_t c = cnt[f]++;
+                    const double* src = arena + b.off_a + j * int64_t(b.nrows) + r0;
+                    float sc = 1.0f;
+                    store_col(f, fbase[f], m, W, c, src, &sc);
+                    const int64_t gc = rcol0[p][f] + c;
+                    L.colsrc[gc] = int32_t(b.col_ofs + j);
+                    if (need_scale) L.zscale[gc] = sc;
+                    L.rdest[gc] = ddest[dstart[s.b] + int64_t(s.pi) * b.ncols + j];
+                }
+            } else {
+                const int64_t e0 = L.blk_t0[s.b];
+                for (int64_t l = 0; l < b.rank; ++l) {
+                    const int f = L.tfmt[e0 + l], W = fmt_width(f);
+                    const int64_t c = cnt[f]++;
+                    const double* src = arena + b.off_a + l * int64_t(b.nrows) + r0;
+                    float sc = 1.0f;
+                    store_col(f, fbase[f], m, W, c, src, &sc);
+                    const int64_t gc = rcol0[p][f] + c;
+                    L.colsrc[gc] = ~int32_t(e0 + l);
+                    if (need_scale) L.zscale[gc] = sc;
+                    L.rdest[gc] = L.qt_off[e0 + l] + s.pi;
+                }
+            }
+        }
+    }
+
+    // ---- fill column panels (V rows, row-major per format) ----
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t q = 0; q < NC; ++q) {
+        const ColPanel& C = L.cp[q];
+        const int64_t m = C.m;
+        int64_t cnt[NFMT] = {0, 0, 0, 0};
+        std::vector<double> col(m);
+        for (const Seg& s : cseg[q]) {
+            const hmvm_block& b = blocks[s.b];
+            const int64_t r0 = C.col0 - b.col_ofs;
+            const int64_t e0 = L.blk_t0[s.b];
+            for (int64_t l = 0; l < b.rank; ++l) {
+                const int f = L.tfmt[e0 + l];
+                const int64_t c = cnt[f]++;
+                const int64_t kp = C.kpad[f];
+                const double* src = arena + b.off_b + l * int64_t(b.ncols) + r0;
+                const int64_t o = C.off[f] + c;
+                float sc = 1.0f;
+                switch (f) {
+                case F64: for (int64_t i = 0; i < m; ++i) L.a64[o + i * kp] = src[i]; break;
+                case F32: for (int64_t i = 0; i < m; ++i) L.a32[o + i * kp] = float(src[i]); break;
+                case BF16: for (int64_t i = 0; i < m; ++i) L.a16[o + i * kp] = encode_bf16(float(src[i])); break;
+                case FP8: {
+                    double mx = 0;
+                    for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
+                    sc = fp8_scale(mx);
+                    for (int64_t i = 0; i < m; ++i) L.a8[o + i * kp] = encode_e4m3(float(src[i] / double(sc)));
+                    break;
+                }
+                }
+                const int64_t vc = C.vcol0[f] + c;
+                L.vdest[vc] = L.pt_off[e0 + l] + s.pi;
+                L.vsrc[vc] = int32_t(e0 + l);
+                if (need_scale) L.vscale[vc] = sc;
+            }
+        }
+    }
+
+    // ---- one-level (TLR) detection: row and column intervals do not nest ----
+    {
+        auto disjoint = [&](bool rows) {
+            std::vector<std::pair<int64_t, int64_t>> iv;
+            for (int64_t i = 0; i < nb; ++i) {
+                const hmvm_block& b = blocks[i];
+                if (rows) iv.push_back({b.row_ofs, b.row_ofs + b.nrows});
+                else iv.push_back({b.col_ofs, b.col_ofs + b.ncols});
+            }
+            std::sort(iv.begin(), iv.end());
+            iv.erase(std::unique(iv.begin(), iv.end()), iv.end());
+            for (size_t t = 1; t < iv.size(); ++t)
+                if (iv[t].first < iv[t - 1].second) return false;
+            return true;
+        };
+        L.one_level = disjoint(true) && disjoint(false);
+    }
+
+    // ---- statistics ----
+    L.bytes_by_format[F64] = int64_t(L.a64.size()) * 8;
+    L.bytes_by_format[F32] = int64_t(L.a32.size()) * 4;
+    L.bytes_by_format[BF16] = int64_t(L.a16.size()) * 2;
+    L.bytes_by_format[FP8] = int64_t(L.a8.size());
+    double fl = 0, ds = 0, br = 0, bv = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+        const hmvm_block& b = blocks[i];
+        const double m = b.nrows, n = b.ncols;
+        if (b.kind == HMVM_BLOCK_DENSE) {
+            fl += 2 * m * n + 2 * m;
+            ds += fmt_bytes(dfmt) * m * n;
+            br += fmt_bytes(dfmt) * m * n;
+        } else {
+            const double k = b.rank;
+            fl += 2 * k * (m + n) + 2 * (m + n);
+            for (int64_t l = 0; l < b.rank; ++l) {
+                const int f = L.tfmt[L.blk_t0[i] + l];
+                ds += fmt_bytes(f) * (m + n);
+                br += fmt_bytes(f) * m;
+                bv += fmt_bytes(f) * n;
+            }
+        }
+    }
+    // per-column scales count as data (4 bytes per stored panel column)
+    if (!L.zscale.empty()) { br += 4.0 * double(L.zscale.size()); bv += 4.0 * double(L.vscale.size()); }
+    L.bytes_rpass = br;
+    L.bytes_vpass = bv;
+    L.flops = fl;
+    L.datasize = ds;
+    return HMVM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// unpack (host copies of the arenas)
+// ---------------------------------------------------------------------------
+
+static double load_elem(int f, const void* const arenas[4], int64_t idx, bool as_f64) {
+    if (as_f64) return static_cast<const double*>(arenas[f])[idx];
+    switch (f) {
+    case F64: return static_cast<const double*>(arenas[0])[idx];
+    case F32: return static_cast<const float*>(arenas[1])[idx];
+    case BF16: return decode_bf16(static_cast<const uint16_t*>(arenas[2])[idx]);
+    default: return decode_e4m3(static_cast<const uint8_t*>(arenas[3])[idx]);
+    }
+}
+
+int unpack_block(const HostLayout& L, int64_t bi, const void* const arenas[4], bool as_f64, double* a, double* bb) {
+    if (bi < 0 || bi >= int64_t(L.blocks.size()) || !a) return HMVM_ERR_INVALID_ARG;
+    const hmvm_block& b = L.blocks[bi];
+    const bool dense = (b.kind == HMVM_BLOCK_DENSE);
+    if (!dense && !bb) return HMVM_ERR_INVALID_ARG;
+    for (int f = 0; f < NFMT; ++f)
+        if (L.bytes_by_format[f] > 0 && !arenas[f]) return HMVM_ERR_INVALID_ARG;
+    const int64_t e0 = L.blk_t0[bi];
+    // R side: walk the row panels the block covers
+    for (size_t p = 0; p < L.rp.size(); ++p) {
+        const RowPanel& P = L.rp[p];
+        if (P.row0 < b.row_ofs || P.row0 >= b.row_ofs + b.nrows) continue;
+        for (int64_t ci = P.chunk0; ci < P.chunk0 + P.nchunks; ++ci) {
+            const RChunk& C = L.rchunks[ci];
+            const int W = fmt_width(C.fmt);
+            for (int64_t c = 0; c < int64_t(C.ngroups) * W; ++c) {
+                const int32_t src = L.colsrc[C.col0 + c];
+                int64_t col = -1;
+                if (dense && src >= 0 && src >= b.col_ofs && src < b.col_ofs + b.ncols) col = src - b.col_ofs;
+                if (!dense && src < 0 && src != kPadCol) {
+                    const int64_t e = ~int64_t(src);
+                    if (e >= e0 && e < e0 + b.rank) col = e - e0;
+                }
+                if (col < 0) continue;
+                const double sc = (as_f64 || L.zscale.empty()) ? 1.0 : L.zscale[C.col0 + c];
+                const int64_t g = c / W, w = c % W;
+                for (int64_t i = 0; i < P.m; ++i) {
+                    const double v = load_elem(C.fmt, arenas, C.data_off + g * P.m * W + i * W + w, as_f64) * sc;
+                    a[col * b.nrows + (P.row0 - b.row_ofs) + i] = v;
+                }
+            }
+        }
+    }
+    if (dense) return HMVM_OK;
+    for (size_t q = 0; q < L.cp.size(); ++q) {
+        const ColPanel& C = L.cp[q];
+        if (C.col0 < b.col_ofs || C.col0 >= b.col_ofs + b.ncols) continue;
+        for (int f = 0; f < NFMT; ++f) {
+            for (int64_t c = 0; c < C.kpad[f]; ++c) {
+                const int32_t e = L.vsrc[C.vcol0[f] + c];
+                if (e < e0 || e >= e0 + b.rank) continue;
+                const int64_t l = e - e0;
+                const double sc = (as_f64 || L.vscale.empty()) ? 1.0 : L.vscale[C.vcol0[f] + c];
+                for (int64_t i = 0; i < C.m; ++i)
+                    bb[l * b.ncols + (C.col0 - b.col_ofs) + i] = load_elem(f, arenas, C.off[f] + i * C.kpad[f] + c, as_f64) * sc;
+            }
+        }
+    }
+    return HMVM_OK;
+}
+
+} // namespace hmvm

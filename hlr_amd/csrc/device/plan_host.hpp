@@ -1,0 +1,77 @@
+// plan_host.hpp -- host-side layout builder of the device plan.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "hmvm.h"
+#include "plan.hpp"
+
+namespace hmvm {
+
+struct HostLayout {
+    int64_t nrows = 0, ncols = 0;
+    int32_t precision = HMVM_PREC_FP64;
+    int32_t dense_fmt = F64;
+    double eps = 0;
+    bool one_level = false;
+    std::vector<hmvm_block> blocks;
+
+    // panels, chunks, work items
+    std::vector<RowPanel> rp;
+    std::vector<RChunk> rchunks;
+    std::vector<ColPanel> cp;
+    std::vector<VItem> vitems;
+
+    // per R column (all row panels, chunk order)
+    std::vector<int32_t> colsrc;   // >= 0: x index; < 0: ~t-entry; kPadCol: padding
+    std::vector<float> zscale;     // multiplier of the gathered value (empty: all 1)
+    std::vector<int32_t> rdest;    // transposed: Q index (-1: padding)
+    // per V column (all column panels, format-major inside a panel)
+    std::vector<int32_t> vdest;    // forward: P index (-1: padding)
+    std::vector<int32_t> vsrc;     // transposed: t-entry (-1: padding)
+    std::vector<float> vscale;     // multiplier of the partial sum (empty: all 1)
+
+    // t entries (one per low-rank column l of block b)
+    int64_t ntent = 0;
+    std::vector<int64_t> blk_t0;   // per block: first t-entry (-1 for dense)
+    std::vector<int8_t> tfmt;      // per t-entry: storage format
+    std::vector<double> tmul;      // per t-entry: S_l for U S V^T leaves, else 1
+    std::vector<int32_t> pt_off, pt_cnt;   // forward partials of t-entry e
+    std::vector<int32_t> qt_off, qt_cnt;   // transposed partials of t-entry e
+    std::vector<int32_t> qd_off, qd_cnt;   // transposed dense partials per column
+    int64_t psize = 0, qsize = 0;
+
+    // factor arenas per format
+    std::vector<double> a64;
+    std::vector<float> a32;
+    std::vector<uint16_t> a16;
+    std::vector<uint8_t> a8;
+
+    // statistics
+    int64_t ndense = 0, nlowrank = 0;
+    int64_t bytes_by_format[NFMT] = {0, 0, 0, 0};
+    int64_t cols_by_format[NFMT] = {0, 0, 0, 0};
+    double flops = 0, datasize = 0, bytes_rpass = 0, bytes_vpass = 0;
+    int32_t max_vitem_rows = 0;
+
+    int64_t arena_elems(int f) const {
+        return f == F64 ? int64_t(a64.size()) : f == F32 ? int64_t(a32.size())
+             : f == BF16 ? int64_t(a16.size()) : int64_t(a8.size());
+    }
+};
+
+// validate the leaves and build the layout; returns an HMVM_* code
+int build_layout(const hmvm_block* blocks, int64_t nblocks, int64_t nrows, int64_t ncols,
+                 const double* arena, int64_t arena_len, const hmvm_plan_opts& opts,
+                 HostLayout& L);
+
+// dequantise one block's factors from (host copies of) the arenas
+int unpack_block(const HostLayout& L, int64_t b, const void* const arenas[4], bool as_f64, double* a, double* bb);
+
+// storage helpers shared with tests
+uint8_t encode_e4m3(float v);
+float decode_e4m3(uint8_t c);
+uint16_t encode_bf16(float v);
+float decode_bf16(uint16_t c);
+
+} // namespace hmvm

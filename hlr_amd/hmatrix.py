@@ -1,0 +1,146 @@
+"""H-/TLR-matrices from the native generator (libhlrgen.so).
+
+`HMatrix` holds the flattened leaf list (in hlr::mul_vec visiting order) and
+the FP64 factor arena produced by hlrgen_build.  The named problems restate
+the workloads the product is measured on (kernel, geometry, sizes, seeds).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from . import _native as N
+
+
+@dataclass(frozen=True)
+class Problem:
+    """Parameters of one generated matrix (mirrors hlrgen_config)."""
+    kernel: int = N.KERNEL_LOG
+    geometry: int = N.GEOM_LINE
+    n: int = 4096
+    seed: int = 1
+    ntile: int = 64
+    format: int = N.FORMAT_H
+    eta: float = 2.0
+    eps: float = 1e-6
+    matern_ell: float = 0.1
+    matern_sigma: float = 1.0
+    nn_diag: int = 8
+    lrformat: int = N.LR_UV
+    nthreads: int = 0
+    tlr_std_adm: int = 0
+
+    def with_(self, **kw) -> "Problem":
+        return replace(self, **kw)
+
+
+# named workloads -----------------------------------------------------------
+
+#: 1D log kernel on n uniform random points in [0,1], leaf 64, eta 2, ACA 1e-6
+LOG1D = Problem(kernel=N.KERNEL_LOG, geometry=N.GEOM_LINE, n=4096)
+#: Matern nu=1.5, l=0.1 on n uniform points in the unit cube
+MATERN_CUBE = Problem(kernel=N.KERNEL_MATERN, geometry=N.GEOM_CUBE, n=65536, matern_ell=0.1)
+#: 1/r on n uniform points on the unit sphere, 8-neighbour diagonal
+INVR_SPHERE = Problem(kernel=N.KERNEL_INVR, geometry=N.GEOM_SPHERE, n=262144, nn_diag=8)
+#: Matern nu=1.5, l=0.1, tile-low-rank with tile 512
+MATERN_TLR = Problem(kernel=N.KERNEL_MATERN, geometry=N.GEOM_CUBE, n=131072, ntile=512,
+                     format=N.FORMAT_TLR)
+
+
+def batch_problems(count: int = 64, seed: int = 7, nmin: int = 2048, nmax: int = 32768):
+    """`count` Matern problems with l ~ U[0.05,0.3], n log-uniform in [nmin,nmax]
+    and nrhs uniform in {1,4,16,64} (seeded); returns [(Problem, nrhs)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        ell = float(rng.uniform(0.05, 0.3))
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        nrhs = int(rng.choice([1, 4, 16, 64]))
+        out.append((Problem(kernel=N.KERNEL_MATERN, geometry=N.GEOM_CUBE, n=n, matern_ell=ell,
+                            seed=1000 + i), nrhs))
+    return out
+
+
+class HMatrix:
+    """A generated H-matrix: leaves + FP64 arena (owned by the native side)."""
+
+    def __init__(self, problem: Problem, verbose: bool = False):
+        lib = N.hlrgen()
+        cfg = N.HlrgenConfig()
+        lib.hlrgen_default_config(C.byref(cfg))
+        for name, _ in N.HlrgenConfig._fields_:
+            if name in ("verbose", "reserved"):
+                continue
+            setattr(cfg, name, getattr(problem, name))
+        cfg.verbose = 1 if verbose else 0
+        h = C.c_void_p()
+        N.check(lib.hlrgen_build(C.byref(cfg), C.byref(h)), "hlrgen_build")
+        self._h = h
+        self.problem = problem
+        self.n = int(lib.hlrgen_n(h))
+        nb = int(lib.hlrgen_nblocks(h))
+        bt = N.block_dtype()
+        raw = (C.c_char * (nb * bt.itemsize)).from_address(lib.hlrgen_blocks(h)) if nb else b""
+        self.blocks = np.frombuffer(raw, dtype=bt, count=nb).copy() if nb else np.zeros(0, bt)
+        alen = int(lib.hlrgen_arena_len(h))
+        self.arena = np.ctypeslib.as_array((C.c_double * alen).from_address(lib.hlrgen_arena(h))) \
+            if alen else np.zeros(0)
+        self.perm = np.ctypeslib.as_array((C.c_int64 * self.n).from_address(lib.hlrgen_perm(h))).copy()
+        self.coords = np.ctypeslib.as_array(
+            (C.c_double * (3 * self.n)).from_address(lib.hlrgen_coords(h))).reshape(self.n, 3).copy()
+        self.build_seconds = float(lib.hlrgen_build_seconds(h))
+
+    # the arena view is only valid while the native matrix lives
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.hlrgen().hlrgen_free(self._h)
+            self._h = None
+            self.arena = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nrows(self) -> int:
+        return self.n
+
+    @property
+    def ncols(self) -> int:
+        return self.n
+
+    def eval(self, rows, cols) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        out = np.zeros((len(rows), len(cols)), order="F")
+        N.check(N.hlrgen().hlrgen_eval(self._h, len(rows), rows.ctypes.data, len(cols), cols.ctypes.data,
+                                       out.ctypes.data), "hlrgen_eval")
+        return out
+
+    def stats(self) -> dict:
+        b = self.blocks
+        dense = b["kind"] == N.BLOCK_DENSE
+        lr = ~dense
+        m = b["nrows"].astype(np.float64)
+        n = b["ncols"].astype(np.float64)
+        k = b["rank"].astype(np.float64)
+        return {
+            "n": self.n, "leaves": int(len(b)), "dense": int(dense.sum()), "lowrank": int(lr.sum()),
+            "kmax": int(k[lr].max()) if lr.any() else 0,
+            "kmean": float(k[lr].mean()) if lr.any() else 0.0,
+            "dense_bytes_fp64": float((m * n)[dense].sum() * 8),
+            "lowrank_bytes_fp64": float((k * (m + n))[lr].sum() * 8),
+            "build_seconds": self.build_seconds,
+        }
+
+
+def random_vector(n: int, seed: int, lo: float = -1.0, hi: float = 1.0, gaussian: bool = False) -> np.ndarray:
+    """Seeded std::mt19937_64 vector: U[lo,hi] or N(0,1)."""
+    out = np.empty(n)
+    N.hlrgen().hlrgen_random_vector(seed, n, lo, hi, 1 if gaussian else 0, out.ctypes.data)
+    return out

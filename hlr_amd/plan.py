@@ -1,0 +1,194 @@
+"""Device plans over the C-ABI (libhmvm.so).
+
+A `Plan` owns the device layout of one H-/TLR-matrix: factor arenas in the
+chosen storage precision plus the panel/index tables.  Vectors are torch
+tensors on the plan's device; torch is used for device memory and streams
+only, all arithmetic runs in the library's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+PRECISIONS = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "valr": N.PREC_VALR}
+FORMAT_NAMES = ("fp64", "fp32", "bf16", "fp8_e4m3")
+FORMAT_BYTES = (8, 4, 2, 1)
+
+
+def _stream_handle(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(int(stream.cuda_stream))
+
+
+def _vec_dtype(t):
+    import torch
+    if t.dtype == torch.float64:
+        return N.F64
+    if t.dtype == torch.float32:
+        return N.F32
+    raise TypeError(f"unsupported vector dtype {t.dtype}")
+
+
+class Plan:
+    """hmvm_plan wrapper.
+
+    blocks: numpy array of hmvm_block records, arena: float64 numpy array.
+    device: HIP device ordinal, or -1 for a host-only layout (no apply).
+    """
+
+    def __init__(self, blocks: np.ndarray, arena: np.ndarray, nrows: int, ncols: int,
+                 precision: str = "fp64", eps: float = 0.0, dense_prec: str = "fp32",
+                 device: int = 0, check_finite: bool = True):
+        self._p = None
+        blocks = np.ascontiguousarray(blocks, dtype=N.block_dtype())
+        arena = np.ascontiguousarray(arena, dtype=np.float64)
+        opts = N.HmvmPlanOpts()
+        opts.precision = PRECISIONS[precision]
+        opts.dense_prec = N.PREC_FP64 if dense_prec == "fp64" else N.PREC_FP32
+        opts.eps = float(eps)
+        opts.check_finite = 1 if check_finite else 0
+        p = C.c_void_p()
+        rc = N.hmvm().hmvm_plan_create(blocks.ctypes.data if len(blocks) else None, len(blocks), nrows, ncols,
+                                       arena.ctypes.data if arena.size else None, arena.size, C.byref(opts),
+                                       device, C.byref(p))
+        N.check(rc, "hmvm_plan_create")
+        self._p = p
+        self.device = device
+        self.precision = precision
+        self.nrows, self.ncols = nrows, ncols
+        self.blocks = blocks
+
+    @classmethod
+    def from_hmatrix(cls, hm, **kw) -> "Plan":
+        return cls(hm.blocks, hm.arena, hm.nrows, hm.ncols, **kw)
+
+    def close(self) -> None:
+        if self._p:
+            N.hmvm().hmvm_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._p
+
+    def info(self) -> dict:
+        i = N.HmvmPlanInfo()
+        N.check(N.hmvm().hmvm_plan_get_info(self._p, C.byref(i)), "hmvm_plan_get_info")
+        d = {name: getattr(i, name) for name, _ in N.HmvmPlanInfo._fields_ if name != "reserved"}
+        d["bytes_by_format"] = list(i.bytes_by_format)
+        d["cols_by_format"] = list(i.cols_by_format)
+        return d
+
+    # -- products -----------------------------------------------------------
+    def apply(self, alpha: float, op: int, x, y, stream=None) -> None:
+        """y += alpha op(A) x for torch tensors (1-D, or 2-D column-major
+        n x nrhs, i.e. a transposed contiguous nrhs x n tensor)."""
+        nrhs, ldx, ldy = 1, 0, 0
+        if x.dim() == 2:
+            # column-major n x nrhs == row-major nrhs x n
+            nrhs = x.shape[0]
+            ldx, ldy = x.stride(0), y.stride(0)
+            if x.stride(1) != 1 or y.stride(1) != 1 or y.shape[0] != nrhs:
+                raise ValueError("x, y must be (nrhs, n) tensors with unit stride along n")
+        rc = N.hmvm().hmvm_apply(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()), ldx,
+                                 C.c_void_p(y.data_ptr()), ldy, nrhs, _vec_dtype(x), _stream_handle(stream))
+        N.check(rc, "hmvm_apply")
+
+    def apply_timed(self, alpha: float, op: int, x, y, stream=None):
+        """apply with HIP events around the kernel phases; returns
+        (reduction pass, partial sums, output pass) durations in ms."""
+        ms = (C.c_float * 3)()
+        rc = N.hmvm().hmvm_apply_timed(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()),
+                                       C.c_void_p(y.data_ptr()), _vec_dtype(x), _stream_handle(stream), ms)
+        N.check(rc, "hmvm_apply_timed")
+        return tuple(float(v) for v in ms)
+
+    def apply_ws(self, alpha, op, x, y, ws, stream=None) -> None:
+        nrhs = x.shape[0] if x.dim() == 2 else 1
+        ldx = x.stride(0) if x.dim() == 2 else 0
+        ldy = y.stride(0) if y.dim() == 2 else 0
+        rc = N.hmvm().hmvm_apply_ws(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()), ldx,
+                                    C.c_void_p(y.data_ptr()), ldy, nrhs, _vec_dtype(x), C.c_void_p(ws.data_ptr()),
+                                    ws.numel() * ws.element_size(), _stream_handle(stream))
+        N.check(rc, "hmvm_apply_ws")
+
+    def workspace_bytes(self, op: int = N.OP_NORMAL, nrhs: int = 1) -> int:
+        b = C.c_int64()
+        N.check(N.hmvm().hmvm_workspace_size(self._p, op, nrhs, C.byref(b)), "hmvm_workspace_size")
+        return int(b.value)
+
+    def tlr_apply(self, alpha: float, op: int, x, y, stream=None) -> None:
+        rc = N.hmvm().hmvm_tlr_apply(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()),
+                                     C.c_void_p(y.data_ptr()), _vec_dtype(x), _stream_handle(stream))
+        N.check(rc, "hmvm_tlr_apply")
+
+    # -- arenas and gradients -------------------------------------------------
+    def arena_elems(self, fmt: int) -> int:
+        e = C.c_int64()
+        N.check(N.hmvm().hmvm_plan_arena_elems(self._p, fmt, C.byref(e)), "hmvm_plan_arena_elems")
+        return int(e.value)
+
+    def arena_numpy(self, fmt: int) -> np.ndarray:
+        """Host copy of the stored arena of format fmt (host-only or device plan)."""
+        dt = (np.float64, np.float32, np.uint16, np.uint8)[fmt]
+        out = np.zeros(self.arena_elems(fmt), dtype=dt)
+        if out.size == 0:
+            return out
+        if self.device < 0:
+            N.check(N.hmvm().hmvm_plan_get_arena(self._p, fmt, C.c_void_p(out.ctypes.data), None), "get_arena")
+            return out
+        import torch
+        t = torch.empty(out.nbytes, dtype=torch.uint8, device=f"cuda:{self.device}")
+        N.check(N.hmvm().hmvm_plan_get_arena(self._p, fmt, C.c_void_p(t.data_ptr()), _stream_handle(None)),
+                "get_arena")
+        out[...] = t.cpu().numpy().view(dt)
+        return out
+
+    def set_arena(self, fmt: int, src) -> None:
+        N.check(N.hmvm().hmvm_plan_set_arena(self._p, fmt, C.c_void_p(src.data_ptr()), _stream_handle(None)),
+                "set_arena")
+
+    def unpack_block(self, b: int, arenas, as_f64: bool = False):
+        """Dequantised (U, V) or (D, None) of block b from host arenas."""
+        blk = self.blocks[b]
+        m, n, k = int(blk["nrows"]), int(blk["ncols"]), int(blk["rank"])
+        dense = int(blk["kind"]) == N.BLOCK_DENSE
+        a = np.zeros((m * n) if dense else (m * k))
+        bb = None if dense else np.zeros(n * k)
+        ptrs = (C.c_void_p * 4)(*[C.c_void_p(arr.ctypes.data) if arr is not None and arr.size else None
+                                  for arr in arenas])
+        N.check(N.hmvm().hmvm_plan_unpack_block(self._p, b, ptrs, 1 if as_f64 else 0, C.c_void_p(a.ctypes.data),
+                                                C.c_void_p(bb.ctypes.data) if bb is not None else None),
+                "hmvm_plan_unpack_block")
+        if dense:
+            return a.reshape(n, m).T, None
+        return a.reshape(k, m).T, bb.reshape(k, n).T
+
+    def grad(self, x, ybar, gx=None, factors: bool = True, stream=None):
+        """Adjoint of y = A x: gx += A^T ybar, and per-format FP64 gradient
+        arenas (dL/d stored factor value, in the plan's layout)."""
+        import torch
+        garena = [None] * 4
+        ptrs = (C.c_void_p * 4)()
+        if factors:
+            for f in range(4):
+                e = self.arena_elems(f)
+                if e:
+                    garena[f] = torch.zeros(e, dtype=torch.float64, device=x.device)
+                    ptrs[f] = C.c_void_p(garena[f].data_ptr())
+        rc = N.hmvm().hmvm_grad(self._p, C.c_void_p(x.data_ptr()), C.c_void_p(ybar.data_ptr()),
+                                C.c_void_p(gx.data_ptr()) if gx is not None else None,
+                                ptrs if factors else None, _vec_dtype(x), _stream_handle(stream))
+        N.check(rc, "hmvm_grad")
+        return garena

@@ -1,0 +1,252 @@
+/*
+ * hmvm.h -- C-ABI of the MI355X H-/TLR-matrix-vector product.
+ *
+ * This is the drop-in boundary for HLR's mul_vec path.  Every entry point is
+ * extern "C", takes plain pointers and sizes, and reports failure through an
+ * int return code (never an exception across the ABI).
+ *
+ * Semantics shared by all entry points (mirroring the reference):
+ *   - accumulate: y += alpha * op(A) * x, y is never zeroed
+ *       (arith/mulvec.hh:27-64, tbb/arith.hh:312-340)
+ *   - alpha == 0 returns immediately without touching y
+ *       (tbb/arith.hh:54-55, tbb/arith.hh:334-335)
+ *   - op in {normal, transposed, adjoint, conjugate}; for real data
+ *     transposed == adjoint and conjugate == normal
+ *       (arith/blas.hh:880-949, matrix/lrmatrix.hh:681-746)
+ *   - vectors are in the internal (cluster) ordering and are addressed by
+ *     global index minus the matrix offset (tbb/detail/arith.hh:168-169)
+ *   - calls are stream ordered and asynchronous; a plan is immutable after
+ *     creation and may be used from several streams at once.
+ */
+#ifndef HMVM_H
+#define HMVM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* error codes                                                              */
+/* ------------------------------------------------------------------------ */
+enum {
+    HMVM_OK                 = 0,
+    HMVM_ERR_INVALID_ARG    = 1,   /* null pointer, bad op, nrhs < 1, ...       */
+    HMVM_ERR_BAD_BLOCK      = 2,   /* offset out of range, rank > min(m,n), ... */
+    HMVM_ERR_OVERLAP        = 3,   /* leaf blocks overlap or leave a hole       */
+    HMVM_ERR_NONFINITE      = 4,   /* NaN/Inf in a factor (cf. check_data())    */
+    HMVM_ERR_ALLOC          = 5,   /* device or host allocation failed          */
+    HMVM_ERR_DEVICE         = 6,   /* HIP runtime error                         */
+    HMVM_ERR_UNSUPPORTED    = 7,   /* combination not implemented               */
+    HMVM_ERR_LAYOUT         = 8,   /* leading dimension too small / misaligned  */
+    HMVM_ERR_KERNEL         = 9    /* a kernel reported a failed internal check */
+};
+
+/* matrix operation, values as Hpro::matop_t */
+enum {
+    HMVM_OP_NORMAL     = 0,
+    HMVM_OP_TRANSPOSED = 1,
+    HMVM_OP_ADJOINT    = 2,
+    HMVM_OP_CONJUGATE  = 3
+};
+
+/* leaf kinds */
+enum {
+    HMVM_BLOCK_DENSE      = 0,   /* D, m x n column-major           (matrix/dense_matrix.hh:36-70) */
+    HMVM_BLOCK_LOWRANK    = 1,   /* U V^H, U m x k, V n x k col-maj (matrix/lrmatrix.hh:39-66)     */
+    HMVM_BLOCK_LOWRANK_SV = 2    /* U diag(S) V^H, S k reals        (matrix/lrsvmatrix.hh:41-61)   */
+};
+
+/* factor storage policy of a plan */
+enum {
+    HMVM_PREC_FP64 = 0,   /* plain path: every factor and accumulation in FP64          */
+    HMVM_PREC_FP32 = 1,   /* every factor FP32, FP32 accumulation inside a panel        */
+    HMVM_PREC_VALR = 2    /* per-column FP64/FP32/BF16/FP8-E4M3 chosen from singular
+                             values against eps (compress/valr.hh:402-512,
+                             compress/afl.hh:81-82,854-856); dense leaves FP32          */
+};
+
+/* vector element type at the boundary */
+enum {
+    HMVM_F64 = 0,
+    HMVM_F32 = 1
+};
+
+/*
+ * One leaf of the flattened block tree.
+ * Replaces the leaf objects of Hpro::TBlockMatrix (arith/mulvec.hh:42-58):
+ * row_ofs/col_ofs are the first indices of row_is()/col_is() in the internal
+ * ordering, off_* are element offsets into the host factor arena.
+ */
+typedef struct hmvm_block {
+    int64_t row_ofs;
+    int64_t col_ofs;
+    int32_t nrows;    /* m                                         */
+    int32_t ncols;    /* n                                         */
+    int32_t rank;     /* k (ignored for dense leaves)              */
+    int32_t kind;     /* HMVM_BLOCK_*                              */
+    int64_t off_a;    /* dense: D; low-rank: U                     */
+    int64_t off_b;    /* low-rank: V                               */
+    int64_t off_s;    /* singular values (k), or -1 when unknown   */
+} hmvm_block;
+
+typedef struct hmvm_plan_opts {
+    int32_t precision;    /* HMVM_PREC_*                                          */
+    int32_t dense_prec;   /* HMVM_PREC_FP64 or HMVM_PREC_FP32 for dense leaves
+                             under HMVM_PREC_VALR (default FP32)                  */
+    double  eps;          /* compression tolerance for HMVM_PREC_VALR             */
+    int32_t check_finite; /* reject NaN/Inf factors at creation (default 1)       */
+    int32_t reserved;
+} hmvm_plan_opts;
+
+typedef struct hmvm_plan_info {
+    int64_t nrows, ncols;
+    int64_t nblocks, ndense, nlowrank;
+    int64_t nrow_panels, ncol_panels;
+    int64_t factor_bytes;        /* device bytes of all stored factors           */
+    int64_t bytes_by_format[4];  /* FP64, FP32, BF16, FP8 factor bytes           */
+    int64_t cols_by_format[4];   /* low-rank columns stored in each format       */
+    int64_t scale_bytes;         /* per-column scale factors                     */
+    double  flops;               /* mul_vec_flops for nrhs = 1                   */
+    double  datasize;            /* mul_vec_datasize restated for this storage   */
+    int64_t index_bytes;         /* per-column source/destination index tables   */
+    int32_t one_level;           /* 1 when the leaves form a one-level tiling    */
+    int32_t reserved;
+    double  bytes_rpass;         /* algorithmic factor bytes of the row-panel
+                                    pass: dense leaves + U factors (+ scales)    */
+    double  bytes_vpass;         /* algorithmic factor bytes of the column-panel
+                                    pass: V factors (+ scales)                   */
+} hmvm_plan_info;
+
+typedef struct hmvm_plan  hmvm_plan;
+typedef struct hmvm_batch hmvm_batch;
+
+/*
+ * Build a device plan from a flattened block tree.
+ * Replaces construction-time preparation that the reference does implicitly
+ * through the TMatrix hierarchy consumed by hlr::tbb::mul_vec
+ * (tbb/arith.hh:312-340); the factor arena is host memory (FP64) that is
+ * converted and uploaded once.  `device` selects the HIP device.
+ */
+int hmvm_plan_create(const hmvm_block* blocks, int64_t nblocks,
+                     int64_t nrows, int64_t ncols,
+                     const double* arena, int64_t arena_len,
+                     const hmvm_plan_opts* opts, int device,
+                     hmvm_plan** out);
+
+int hmvm_plan_destroy(hmvm_plan* plan);
+
+int hmvm_plan_get_info(const hmvm_plan* plan, hmvm_plan_info* info);
+
+/*
+ * y += alpha * op(A) * x  for nrhs right-hand sides.
+ * Replaces hlr::tbb::mul_vec (tbb/arith.hh:312-340) for nrhs == 1 and the
+ * multi-RHS leaf apply_add (matrix/lrmatrix.hh:681-746,
+ * matrix/dense_matrix.hh:483-496) lifted to the whole H-matrix for nrhs > 1.
+ * x and y are device pointers, column-major with leading dimensions ldx/ldy
+ * (in elements) of type vec_dtype.  Temporaries are stream-ordered
+ * allocations on `stream` (a hipStream_t, NULL = default stream).
+ */
+int hmvm_apply(const hmvm_plan* plan, double alpha, int op,
+               const void* x, int64_t ldx, void* y, int64_t ldy,
+               int nrhs, int vec_dtype, void* stream);
+
+/*
+ * hmvm_apply for one right-hand side that also records HIP events on
+ * `stream` around its three kernel phases and, after synchronising the
+ * stream, returns their durations in milliseconds:
+ *   ms[0]  reduction pass   (forward: V^T x per column panel;
+ *                            transposed: R^T x per row panel)
+ *   ms[1]  partial sums     (t = alpha S sum P, or s = alpha S sum Q)
+ *   ms[2]  output pass      (forward: y += R z; transposed: y += V s + dense)
+ * For measurement; no counterpart in the reference.
+ */
+int hmvm_apply_timed(const hmvm_plan* plan, double alpha, int op,
+                     const void* x, void* y, int vec_dtype, void* stream,
+                     float ms[3]);
+
+/* bytes of workspace hmvm_apply_ws needs for (op, nrhs) */
+int hmvm_workspace_size(const hmvm_plan* plan, int op, int nrhs, int64_t* bytes);
+
+/* hmvm_apply with a caller-provided device workspace (no allocation) */
+int hmvm_apply_ws(const hmvm_plan* plan, double alpha, int op,
+                  const void* x, int64_t ldx, void* y, int64_t ldy,
+                  int nrhs, int vec_dtype, void* workspace, int64_t ws_bytes,
+                  void* stream);
+
+/*
+ * Tile-low-rank product.  Replaces hlr::tlr::mul_vec (arith/mulvec.hh:882-919,
+ * tbb/arith.hh:998-1042).  A TLR matrix is an H-matrix with a one-level tree,
+ * so this is hmvm_apply on a plan whose blocks form a one-level tiling; it
+ * fails with HMVM_ERR_INVALID_ARG when the plan is not one-level.
+ */
+int hmvm_tlr_apply(const hmvm_plan* plan, double alpha, int op,
+                   const void* x, void* y, int vec_dtype, void* stream);
+
+/*
+ * Adjoint of y = A x with respect to x and to every factor.
+ * Given x and the output cotangent ybar (device, FP64 or FP32 per vec_dtype):
+ *   gx     += A^T ybar                          (transposed H-MVM)
+ *   dU_b    = ybar_b (V_b^T x_b)^T              (V^T x recomputed, not stored)
+ *   dV_b    = x_b (U_b^T ybar_b)^T
+ *   dD_b    = ybar_b x_b^T
+ * Gradients are written into device buffers in the plan's own arena layout
+ * (one per storage format, see hmvm_plan_arena_elems); pass NULL to skip.
+ * No counterpart in the reference (it has no differentiation); the transposed
+ * product follows arith/blas.hh:910-948.
+ */
+int hmvm_grad(const hmvm_plan* plan, const void* x, const void* ybar,
+              void* gx, void* grad_arena[4], int vec_dtype, void* stream);
+
+/* number of elements of the device arena of storage format f (0..3) */
+int hmvm_plan_arena_elems(const hmvm_plan* plan, int fmt, int64_t* elems);
+
+/* copy a device arena out of / into the plan (device to device) */
+int hmvm_plan_get_arena(const hmvm_plan* plan, int fmt, void* dst, void* stream);
+int hmvm_plan_set_arena(hmvm_plan* plan, int fmt, const void* src, void* stream);
+
+/*
+ * Expand one block's factors from host copies of the plan arenas back into
+ * FP64 column-major form (U m x k into `a` and V n x k into `b`, or D m x n
+ * into `a`), undoing the panel layout.  as_f64 == 0: the arenas hold stored
+ * factors (decoded and multiplied by their per-column scales); as_f64 != 0:
+ * every arena holds FP64 values in the same layout, e.g. gradients from
+ * hmvm_grad, taken as they are.
+ */
+int hmvm_plan_unpack_block(const hmvm_plan* plan, int64_t block,
+                           const void* const host_arenas[4], int as_f64,
+                           double* a, double* b);
+
+/* ------------------------------------------------------------------------ */
+/* batching: many plans, many right-hand sides, one persistent launch       */
+/* ------------------------------------------------------------------------ */
+
+int hmvm_batch_create(const hmvm_plan* const* plans, int nplans,
+                      const int* nrhs, int vec_dtype, hmvm_batch** out);
+int hmvm_batch_destroy(hmvm_batch* batch);
+
+/* y_i += alpha_i A_i x_i for every plan i, in one persistent launch */
+int hmvm_batch_apply(hmvm_batch* batch, const double* alphas,
+                     const void* const* xs, const int64_t* ldx,
+                     void* const* ys, const int64_t* ldy, void* stream);
+
+/* capture hmvm_batch_apply into a HIP graph (pointers fixed), replay it */
+int hmvm_batch_capture(hmvm_batch* batch, const double* alphas,
+                       const void* const* xs, const int64_t* ldx,
+                       void* const* ys, const int64_t* ldy, void* stream);
+int hmvm_batch_replay(hmvm_batch* batch, void* stream);
+
+/* time candidate launch shapes on `stream` and keep the fastest */
+int hmvm_batch_tune(hmvm_batch* batch, const void* const* xs,
+                    const int64_t* ldx, void* const* ys,
+                    const int64_t* ldy, void* stream, int* chosen);
+
+const char* hmvm_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMVM_H */

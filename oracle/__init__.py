@@ -1,0 +1,147 @@
+"""Oracle: CPU FP64 restatement of HLR's mul_vec path (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this package, and only as the checker or the timed CPU baseline.
+Each function cites the reference code it restates in oracle/hmvm_oracle.c.
+
+Parity status of the functions (what pins each one is listed in the C file
+header and exercised by tests/test_oracle.py):
+  mul_vec, tlr_mul_vec, mul_vec_chunk      pinned (dense expansion, brute force,
+                                           invariants)
+  mul_vec_flops, mul_vec_datasize          pinned (hand-counted trees)
+  valr_tolerances, eps_to_rate             pinned (closed forms on synthetic
+                                           singular values)
+  round_bf16, round_e4m3                   pinned (torch dtype conversions and
+                                           the OCP E4M3 code table)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB = None
+_PATH = Path(__file__).resolve().parent / "liboracle.so"
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not _PATH.exists():
+            from . import build as _b
+            _b.build()
+        L = C.CDLL(str(_PATH))
+        vp, i64, i32, d = C.c_void_p, C.c_int64, C.c_int, C.c_double
+        L.oracle_mul_vec.argtypes = [vp, i64, vp, d, i32, vp, i64, vp, i64, i32]
+        L.oracle_mul_vec.restype = i32
+        L.oracle_tlr_mul_vec.argtypes = [vp, i64, vp, d, i32, vp, vp, i32]
+        L.oracle_tlr_mul_vec.restype = i32
+        L.oracle_mul_vec_chunk.argtypes = [vp, i64, vp, d, i32, vp, vp, i64, i32]
+        L.oracle_mul_vec_chunk.restype = i32
+        L.oracle_time_chunk.argtypes = [vp, i64, vp, vp, vp, i64, i32, i32]
+        L.oracle_time_chunk.restype = d
+        L.oracle_mul_vec_flops.argtypes = [vp, i64, i32]
+        L.oracle_mul_vec_flops.restype = d
+        L.oracle_mul_vec_datasize.argtypes = [vp, i64, i64, i64]
+        L.oracle_mul_vec_datasize.restype = d
+        L.oracle_dense_expand.argtypes = [vp, i64, vp, i64, i64, vp]
+        L.oracle_dense_expand.restype = i32
+        L.oracle_valr_tolerances.argtypes = [i64, vp, d, vp]
+        L.oracle_valr_tolerances.restype = None
+        L.oracle_eps_to_rate.argtypes = [d]
+        L.oracle_eps_to_rate.restype = i32
+        L.oracle_format_for_rate.argtypes = [i32]
+        L.oracle_format_for_rate.restype = i32
+        L.oracle_round_bf16.argtypes = [d]
+        L.oracle_round_bf16.restype = d
+        L.oracle_encode_e4m3.argtypes = [d]
+        L.oracle_encode_e4m3.restype = C.c_uint8
+        L.oracle_decode_e4m3.argtypes = [C.c_uint8]
+        L.oracle_decode_e4m3.restype = d
+        L.oracle_round_e4m3.argtypes = [d]
+        L.oracle_round_e4m3.restype = d
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def mul_vec(blocks, arena, alpha, op, x, y):
+    """y += alpha op(A) x in the sequential leaf order (y modified in place).
+    x, y: 1-D, or 2-D (nrhs, n) C-contiguous arrays."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    assert y.dtype == np.float64 and y.flags.c_contiguous
+    nrhs = x.shape[0] if x.ndim == 2 else 1
+    ldx = x.shape[-1]
+    ldy = y.shape[-1]
+    rc = lib().oracle_mul_vec(_p(blocks), len(blocks), _p(arena), float(alpha), int(op), _p(x), ldx, _p(y), ldy, nrhs)
+    assert rc == 0, rc
+    return y
+
+
+def tlr_mul_vec(blocks, arena, alpha, op, x, y, nthreads=1):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rc = lib().oracle_tlr_mul_vec(_p(blocks), len(blocks), _p(arena), float(alpha), int(op), _p(x), _p(y), nthreads)
+    assert rc == 0, rc
+    return y
+
+
+def mul_vec_chunk(blocks, arena, alpha, op, x, y, nthreads=1):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rc = lib().oracle_mul_vec_chunk(_p(blocks), len(blocks), _p(arena), float(alpha), int(op), _p(x), _p(y),
+                                    len(y), nthreads)
+    assert rc == 0, rc
+    return y
+
+
+def time_chunk(blocks, arena, x, y, nthreads, nmvm):
+    return lib().oracle_time_chunk(_p(blocks), len(blocks), _p(arena), _p(x), _p(y), len(y), nthreads, nmvm)
+
+
+def mul_vec_flops(blocks, op=0):
+    return lib().oracle_mul_vec_flops(_p(blocks), len(blocks), op)
+
+
+def mul_vec_datasize(blocks, nrows, ncols):
+    return lib().oracle_mul_vec_datasize(_p(blocks), len(blocks), nrows, ncols)
+
+
+def dense_expand(blocks, arena, nrows, ncols):
+    A = np.zeros((ncols, nrows))      # column-major nrows x ncols
+    rc = lib().oracle_dense_expand(_p(blocks), len(blocks), _p(arena), nrows, ncols, _p(A))
+    assert rc == 0, rc
+    return A.T
+
+
+def valr_tolerances(sv, eps):
+    sv = np.ascontiguousarray(sv, dtype=np.float64)
+    out = np.zeros_like(sv)
+    lib().oracle_valr_tolerances(len(sv), _p(sv), float(eps), _p(out))
+    return out
+
+
+def eps_to_rate(eps):
+    return lib().oracle_eps_to_rate(float(eps))
+
+
+def format_for_rate(rate):
+    return lib().oracle_format_for_rate(int(rate))
+
+
+def round_bf16(v):
+    return lib().oracle_round_bf16(float(v))
+
+
+def encode_e4m3(v):
+    return lib().oracle_encode_e4m3(float(v))
+
+
+def decode_e4m3(c):
+    return lib().oracle_decode_e4m3(int(c))
+
+
+def round_e4m3(v):
+    return lib().oracle_round_e4m3(float(v))

@@ -1,0 +1,410 @@
+/*
+ * hmvm_oracle.c -- CPU FP64 restatement of HLR's mul_vec path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker or the CPU baseline -- never as the product path.
+ *
+ * Functions and the reference code each one restates:
+ *   oracle_mul_vec        hlr::mul_vec / seq::mul_vec, leaf order of the
+ *                         row-major recursion (arith/mulvec.hh:27-64,72-81;
+ *                         seq/arith.hh:40-72), leaves via apply_add
+ *                         (matrix/lrmatrix.hh:625-679,
+ *                         matrix/dense_matrix.hh:459-481) and mulvec_lr
+ *                         (arith/blas.hh:880-949, 1030-1103)
+ *   oracle_tlr_mul_vec    hlr::tlr::mul_vec (arith/mulvec.hh:882-919) and the
+ *                         TBB variant with a private t_i per block row
+ *                         (tbb/arith.hh:998-1042)
+ *   oracle_mul_vec_chunk  hlr::tbb::mul_vec_chunk: parallel leaves, private
+ *                         yt per leaf, 64-entry locked chunk updates
+ *                         (tbb/arith.hh:46-61; tbb/detail/arith.hh:21-23,
+ *                         95-175)
+ *   oracle_mul_vec_flops  mul_vec_flops (arith/mulvec.hh:795-862) with
+ *                         FLOPS_DGEMV (utils/flops.hh:41-42,212)
+ *   oracle_mul_vec_datasize  mul_vec_datasize (arith/mulvec.hh:867-874) with
+ *                         data_byte_size (matrix/lrmatrix.hh:552-558,
+ *                         matrix/dense_matrix.hh:389-395)
+ *   oracle_valr_tolerances  valr::get_tolerances (compress/valr.hh:402-512)
+ *   oracle_eps_to_rate    afl::eps_to_rate (compress/afl.hh:81-82)
+ *   oracle_round_*        round-to-nearest-even into BF16 / FP8-E4M3 (OCP)
+ *   oracle_dense_expand   sum of the leaves placed at their offsets
+ *
+ * Pins (tests/): dense expansion against direct kernel evaluation and
+ * numpy products; linearity, accumulate, alpha = 0, <Ax,w> = <x,A^T w>,
+ * TLR == H on one-level trees, brute force on hand-built trees; rounding
+ * against torch's float8_e4m3fn / bfloat16 conversions and the OCP table.
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "hmvm.h"
+
+#define ORACLE_CHUNK 64
+
+/* y(m) += alpha * A(m x n, col-major, lda) * x(n) -- netlib dgemv 'N' order */
+static void gemv_n(int64_t m, int64_t n, double alpha, const double* A, int64_t lda,
+                   const double* x, double* y) {
+    for (int64_t j = 0; j < n; ++j) {
+        const double t = alpha * x[j];
+        if (t == 0.0) continue;
+        const double* a = A + j * lda;
+        for (int64_t i = 0; i < m; ++i) y[i] += t * a[i];
+    }
+}
+
+/* y(n) += alpha * A(m x n)^T * x(m) -- netlib dgemv 'T' order */
+static void gemv_t(int64_t m, int64_t n, double alpha, const double* A, int64_t lda,
+                   const double* x, double* y) {
+    for (int64_t j = 0; j < n; ++j) {
+        const double* a = A + j * lda;
+        double t = 0.0;
+        for (int64_t i = 0; i < m; ++i) t += a[i] * x[i];
+        y[j] += alpha * t;
+    }
+}
+
+/* leaf apply_add(alpha, x, y, op) for real data (transposed == adjoint) */
+static void leaf_apply(const hmvm_block* b, const double* arena, double alpha, int op,
+                       const double* x, double* y, double* t) {
+    const int64_t m = b->nrows, n = b->ncols, k = b->rank;
+    const int trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
+    if (b->kind == HMVM_BLOCK_DENSE) {
+        const double* D = arena + b->off_a;
+        if (!trans) gemv_n(m, n, alpha, D, m, x, y);
+        else        gemv_t(m, n, alpha, D, m, x, y);
+        return;
+    }
+    const double* U = arena + b->off_a;
+    const double* V = arena + b->off_b;
+    const double* S = (b->kind == HMVM_BLOCK_LOWRANK_SV) ? arena + b->off_s : NULL;
+    if (k == 0) return;
+    memset(t, 0, sizeof(double) * (size_t)k);
+    if (!trans) {
+        /* t := V^H x ; t := alpha (S) t ; y := y + U t */
+        gemv_t(n, k, 1.0, V, n, x, t);
+        for (int64_t l = 0; l < k; ++l) t[l] *= S ? alpha * S[l] : alpha;
+        gemv_n(m, k, 1.0, U, m, t, y);
+    } else {
+        /* t := U^T x ; t := alpha (S) t ; y := y + V t */
+        gemv_t(m, k, 1.0, U, m, x, t);
+        for (int64_t l = 0; l < k; ++l) t[l] *= S ? alpha * S[l] : alpha;
+        gemv_n(n, k, 1.0, V, n, t, y);
+    }
+}
+
+static int64_t max_rank(const hmvm_block* blocks, int64_t nb) {
+    int64_t k = 1;
+    for (int64_t i = 0; i < nb; ++i)
+        if (blocks[i].kind != HMVM_BLOCK_DENSE && blocks[i].rank > k) k = blocks[i].rank;
+    return k;
+}
+
+static int64_t max_dim(const hmvm_block* blocks, int64_t nb) {
+    int64_t d = 1;
+    for (int64_t i = 0; i < nb; ++i) {
+        if (blocks[i].nrows > d) d = blocks[i].nrows;
+        if (blocks[i].ncols > d) d = blocks[i].ncols;
+    }
+    return d;
+}
+
+/*
+ * y += alpha op(A) x, sequential, leaves in the given (recursion) order.
+ * x and y hold nrhs columns with leading dimensions ldx / ldy.
+ */
+int oracle_mul_vec(const hmvm_block* blocks, int64_t nb, const double* arena, double alpha,
+                   int op, const double* x, int64_t ldx, double* y, int64_t ldy, int nrhs) {
+    if (alpha == 0.0) return 0;
+    const int trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
+    double* t = (double*)malloc(sizeof(double) * (size_t)max_rank(blocks, nb));
+    if (!t) return HMVM_ERR_ALLOC;
+    for (int r = 0; r < nrhs; ++r)
+        for (int64_t i = 0; i < nb; ++i) {
+            const hmvm_block* b = &blocks[i];
+            const int64_t xo = trans ? b->row_ofs : b->col_ofs;
+            const int64_t yo = trans ? b->col_ofs : b->row_ofs;
+            leaf_apply(b, arena, alpha, op, x + r * ldx + xo, y + r * ldy + yo, t);
+        }
+    free(t);
+    return 0;
+}
+
+/*
+ * TLR product: per block row i a private t_i, accumulated over the block
+ * columns and then added to y_i (tbb/arith.hh:998-1042).  Block rows are
+ * identified by row_ofs (op normal) or col_ofs (transposed); leaves must be
+ * grouped by block row, which the one-level tilings are (row-major order).
+ * nthreads > 1 runs block rows in parallel like the TBB version.
+ */
+int oracle_tlr_mul_vec(const hmvm_block* blocks, int64_t nb, const double* arena, double alpha,
+                       int op, const double* x, double* y, int nthreads) {
+    if (alpha == 0.0) return 0;
+    const int trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
+    /* group leaves by output block: for op normal the list is already grouped,
+       for transposed build column groups */
+    int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb ? nb : 1));
+    int64_t* gstart = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    if (!order || !gstart) { free(order); free(gstart); return HMVM_ERR_ALLOC; }
+    int64_t ng = 0, pos = 0;
+    char* done = (char*)calloc((size_t)(nb ? nb : 1), 1);
+    if (!done) { free(order); free(gstart); return HMVM_ERR_ALLOC; }
+    for (int64_t i = 0; i < nb; ++i) {
+        if (done[i]) continue;
+        const int64_t key = trans ? blocks[i].col_ofs : blocks[i].row_ofs;
+        gstart[ng++] = pos;
+        for (int64_t j = i; j < nb; ++j) {
+            const int64_t kj = trans ? blocks[j].col_ofs : blocks[j].row_ofs;
+            if (!done[j] && kj == key) { order[pos++] = j; done[j] = 1; }
+        }
+    }
+    gstart[ng] = pos;
+    free(done);
+    const int64_t kmax = max_rank(blocks, nb), dmax = max_dim(blocks, nb);
+    int err = 0;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        double* t = (double*)malloc(sizeof(double) * (size_t)kmax);
+        double* ti = (double*)malloc(sizeof(double) * (size_t)dmax);
+        if (!t || !ti) {
+#pragma omp atomic write
+            err = HMVM_ERR_ALLOC;
+        } else {
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t g = 0; g < ng; ++g) {
+                const hmvm_block* b0 = &blocks[order[gstart[g]]];
+                const int64_t len = trans ? b0->ncols : b0->nrows;
+                const int64_t yo = trans ? b0->col_ofs : b0->row_ofs;
+                memset(ti, 0, sizeof(double) * (size_t)len);
+                for (int64_t p = gstart[g]; p < gstart[g + 1]; ++p) {
+                    const hmvm_block* b = &blocks[order[p]];
+                    const int64_t xo = trans ? b->row_ofs : b->col_ofs;
+                    leaf_apply(b, arena, alpha, op, x + xo, ti, t);
+                }
+                for (int64_t i = 0; i < len; ++i) y[yo + i] += ti[i];
+            }
+        }
+        free(t);
+        free(ti);
+    }
+    free(order);
+    free(gstart);
+    return err;
+}
+
+/*
+ * Restatement of tbb::mul_vec_chunk: every leaf computes yt = alpha op(B) x
+ * into a private temporary and adds it to y in 64-entry chunks, each chunk
+ * under its own lock.  The lock vector is allocated per call, as in the
+ * reference.  Summation order across leaves is not deterministic.
+ */
+int oracle_mul_vec_chunk(const hmvm_block* blocks, int64_t nb, const double* arena, double alpha,
+                         int op, const double* x, double* y, int64_t ny, int nthreads) {
+    if (alpha == 0.0) return 0;
+    const int trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
+    const int64_t nchunks = ny / ORACLE_CHUNK + 1;
+    omp_lock_t* locks = (omp_lock_t*)malloc(sizeof(omp_lock_t) * (size_t)nchunks);
+    if (!locks) return HMVM_ERR_ALLOC;
+    for (int64_t c = 0; c < nchunks; ++c) omp_init_lock(&locks[c]);
+    const int64_t kmax = max_rank(blocks, nb), dmax = max_dim(blocks, nb);
+    int err = 0;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+        double* t = (double*)malloc(sizeof(double) * (size_t)kmax);
+        double* yt = (double*)malloc(sizeof(double) * (size_t)dmax);
+        if (!t || !yt) {
+#pragma omp atomic write
+            err = HMVM_ERR_ALLOC;
+        } else {
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t i = 0; i < nb; ++i) {
+                const hmvm_block* b = &blocks[i];
+                const int64_t xo = trans ? b->row_ofs : b->col_ofs;
+                const int64_t yo = trans ? b->col_ofs : b->row_ofs;
+                const int64_t len = trans ? b->ncols : b->nrows;
+                memset(yt, 0, sizeof(double) * (size_t)len);
+                leaf_apply(b, arena, alpha, op, x + xo, yt, t);
+                /* update(): chunks of the global index space */
+                int64_t start = yo;
+                const int64_t last = yo + len - 1;
+                int64_t chunk = start / ORACLE_CHUNK;
+                int64_t end = (chunk + 1) * ORACLE_CHUNK - 1;
+                if (end > last) end = last;
+                while (start <= end) {
+                    omp_set_lock(&locks[chunk]);
+                    for (int64_t g = start; g <= end; ++g) y[g] += yt[g - yo];
+                    omp_unset_lock(&locks[chunk]);
+                    ++chunk;
+                    start = end + 1;
+                    end = end + ORACLE_CHUNK;
+                    if (end > last) end = last;
+                }
+            }
+        }
+        free(t);
+        free(yt);
+    }
+    for (int64_t c = 0; c < nchunks; ++c) omp_destroy_lock(&locks[c]);
+    free(locks);
+    return err;
+}
+
+/* seconds for nmvm calls of oracle_mul_vec_chunk (alpha = 2, x given) */
+double oracle_time_chunk(const hmvm_block* blocks, int64_t nb, const double* arena,
+                         const double* x, double* y, int64_t ny, int nthreads, int nmvm) {
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int i = 0; i < nmvm; ++i)
+        oracle_mul_vec_chunk(blocks, nb, arena, 2.0, HMVM_OP_NORMAL, x, y, ny, nthreads);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* FLOPS_DGEMV(m, n) = m n + 2 m (mults) + m n (adds) */
+static double flops_dgemv(double m, double n) { return 2.0 * m * n + 2.0 * m; }
+
+double oracle_mul_vec_flops(const hmvm_block* blocks, int64_t nb, int op) {
+    const int trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
+    double f = 0.0;
+    for (int64_t i = 0; i < nb; ++i) {
+        const double m = trans ? blocks[i].ncols : blocks[i].nrows;
+        const double n = trans ? blocks[i].nrows : blocks[i].ncols;
+        if (blocks[i].kind == HMVM_BLOCK_DENSE) f += flops_dgemv(m, n);
+        else f += flops_dgemv(n, blocks[i].rank) + flops_dgemv(m, blocks[i].rank);
+    }
+    return f;
+}
+
+/* data bytes of the leaves + 2 sizeof(double) (nrows + ncols) */
+double oracle_mul_vec_datasize(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t ncols) {
+    double s = 0.0;
+    for (int64_t i = 0; i < nb; ++i) {
+        const double m = blocks[i].nrows, n = blocks[i].ncols;
+        if (blocks[i].kind == HMVM_BLOCK_DENSE) s += 8.0 * m * n;
+        else s += 8.0 * blocks[i].rank * (m + n);
+    }
+    return s + 2.0 * 8.0 * (double)(nrows + ncols);
+}
+
+/* A (nrows x ncols col-major) = sum of leaves at their offsets */
+int oracle_dense_expand(const hmvm_block* blocks, int64_t nb, const double* arena,
+                        int64_t nrows, int64_t ncols, double* A) {
+    memset(A, 0, sizeof(double) * (size_t)(nrows * ncols));
+    for (int64_t i = 0; i < nb; ++i) {
+        const hmvm_block* b = &blocks[i];
+        const int64_t m = b->nrows, n = b->ncols, k = b->rank;
+        if (b->row_ofs < 0 || b->col_ofs < 0 || b->row_ofs + m > nrows || b->col_ofs + n > ncols)
+            return HMVM_ERR_BAD_BLOCK;
+        for (int64_t j = 0; j < n; ++j) {
+            double* a = A + (b->col_ofs + j) * nrows + b->row_ofs;
+            if (b->kind == HMVM_BLOCK_DENSE) {
+                const double* d = arena + b->off_a + j * m;
+                for (int64_t r = 0; r < m; ++r) a[r] += d[r];
+            } else {
+                for (int64_t l = 0; l < k; ++l) {
+                    double v = arena[b->off_b + l * n + j];
+                    if (b->kind == HMVM_BLOCK_LOWRANK_SV) v *= arena[b->off_s + l];
+                    const double* u = arena + b->off_a + l * m;
+                    for (int64_t r = 0; r < m; ++r) a[r] += u[r] * v;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/*
+ * valr::get_tolerances for relative accuracy eps in the spectral norm
+ * (norm = sv(0)), including the correction that solves for eps' with
+ * sum(1/sigma) eps'^2 + eps' = delta via p/q (compress/valr.hh:456-499);
+ * returns tol / sigma_l per column.
+ */
+void oracle_valr_tolerances(int64_t k, const double* sv, double eps, double* out) {
+    if (k <= 0) return;
+    double tol = eps * sv[0];
+    double sum = 0.0;
+    for (int64_t i = 0; i < k; ++i) sum += 1.0 / sv[i];
+    const double p = 1.0 / sum, q = tol / sum;
+    double t = p * p / 4.0 - q;
+    if (t < 0) t = 0;
+    const double d1 = fabs(p / 2.0 + sqrt(t)), d2 = fabs(p / 2.0 - sqrt(t));
+    tol = (d1 < d2) ? d1 : d2;
+    for (int64_t l = 0; l < k; ++l) out[l] = tol / sv[l];
+}
+
+/* mantissa bits needed for relative accuracy eps: max(1, ceil(-log2 eps)) */
+int oracle_eps_to_rate(double eps) {
+    const double r = ceil(-log2(eps));
+    return (int)(r < 1.0 ? 1.0 : r);
+}
+
+/* storage format for a column with mantissa-bit requirement `rate`:
+   0 = FP64 (52), 1 = FP32 (23), 2 = BF16 (7), 3 = FP8-E4M3 (3) */
+int oracle_format_for_rate(int rate) {
+    if (rate <= 3) return 3;
+    if (rate <= 7) return 2;
+    if (rate <= 23) return 1;
+    return 0;
+}
+
+/* round-to-nearest-even of a double to BF16, returned as double */
+double oracle_round_bf16(double v) {
+    const float f = (float)v;              /* the device converts from FP32 too */
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return f;   /* inf / nan unchanged */
+    const uint32_t lsb = (u >> 16) & 1u;
+    u = (u + 0x7fffu + lsb) & 0xffff0000u;
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+/*
+ * OCP FP8 E4M3: 1 sign, 4 exponent (bias 7), 3 mantissa bits, no infinity,
+ * max finite 448, subnormals down to 2^-9.  Round to nearest even,
+ * saturating at +-448.  Returns the 8-bit code.
+ */
+uint8_t oracle_encode_e4m3(double v) {
+    const float f = (float)v;
+    uint8_t sign = (f < 0 || (f == 0 && signbit(f))) ? 0x80 : 0x00;
+    double a = fabs((double)f);
+    if (isnan(f)) return 0x7f;
+    if (a >= 464.0) return sign | 0x7e;                 /* saturate (448 = 0x7e) */
+    /* grid spacing for |a|: exponent e with 2^e <= a < 2^(e+1), e >= -6 */
+    int e = (a > 0) ? (int)floor(log2(a)) : -6;
+    if (e < -6) e = -6;
+    const double q = ldexp(1.0, e - 3);                  /* spacing */
+    double r = a / q;
+    double fl = floor(r);
+    double diff = r - fl;
+    double mant;
+    if (diff > 0.5) mant = fl + 1;
+    else if (diff < 0.5) mant = fl;
+    else mant = (fmod(fl, 2.0) == 0.0) ? fl : fl + 1;
+    double val = mant * q;
+    if (val > 448.0) val = 448.0;
+    /* encode val */
+    if (val == 0.0) return sign;
+    int ee = (int)floor(log2(val));
+    if (ee < -6) {                                       /* subnormal: m * 2^-9 */
+        const int m = (int)(val / ldexp(1.0, -9) + 0.5);
+        return sign | (uint8_t)m;
+    }
+    const int m = (int)(val / ldexp(1.0, ee - 3) + 0.5) - 8;
+    return sign | (uint8_t)(((ee + 7) << 3) | m);
+}
+
+double oracle_decode_e4m3(uint8_t c) {
+    const int s = c >> 7, e = (c >> 3) & 0xf, m = c & 7;
+    double v;
+    if (e == 0xf && m == 7) return NAN;
+    if (e == 0) v = ldexp((double)m, -9);
+    else v = ldexp(1.0 + m / 8.0, e - 7);
+    return s ? -v : v;
+}
+
+double oracle_round_e4m3(double v) { return oracle_decode_e4m3(oracle_encode_e4m3(v)); }
