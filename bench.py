@@ -45,6 +45,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--precision", choices=["both", "fp64", "valr"], default="both",
+                    help="time only one plan (profiling runs); the JSON line needs both")
+    ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -96,27 +99,56 @@ def main() -> int:
         torch.cuda.synchronize()
         return y.cpu().numpy()
 
-    # ---- plain FP64 plan ----
-    t0 = time.time()
-    p64 = Plan.from_hmatrix(hm, precision="fp64")
-    i64 = p64.info()
-    log(f"fp64 plan: {i64['factor_bytes'] / 1e9:.2f} GB in {time.time() - t0:.1f} s")
-    ms64, ph64 = time_plan(p64, args.steps, args.warmup)
-    y64 = result(p64)
-    log(f"fp64: {ms64:.3f} ms/MVM  phases {ph64.round(4).tolist()} ms")
-    p64.close()
+    # ---- device copy of known size: achievable HBM bandwidth on this GPU ----
+    nbytes = int(args.copy_gib * 2 ** 30)
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    c0 = torch.cuda.Event(enable_timing=True)
+    c1 = torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(10):
+        dst.copy_(src)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2.0 * nbytes / (c0.elapsed_time(c1) / 10 * 1e-3) / 1e9
+    del src, dst
     torch.cuda.empty_cache()
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write of {args.copy_gib:g} GiB)")
+
+    ms64 = msmp = None
+    ph64 = phmp = np.zeros(3)
+    i64 = imp = None
+    y64 = ymp = None
+    # ---- plain FP64 plan ----
+    if args.precision in ("both", "fp64"):
+        t0 = time.time()
+        p64 = Plan.from_hmatrix(hm, precision="fp64")
+        i64 = p64.info()
+        log(f"fp64 plan: {i64['factor_bytes'] / 1e9:.2f} GB in {time.time() - t0:.1f} s")
+        ms64, ph64 = time_plan(p64, args.steps, args.warmup)
+        y64 = result(p64)
+        log(f"fp64: {ms64:.3f} ms/MVM  phases {ph64.round(4).tolist()} ms")
+        p64.close()
+        torch.cuda.empty_cache()
 
     # ---- mixed-precision plan ----
-    t0 = time.time()
-    pmp = Plan.from_hmatrix(hm, precision="valr", eps=args.eps)
-    imp = pmp.info()
-    log(f"valr plan: {imp['factor_bytes'] / 1e9:.2f} GB cols/format {imp['cols_by_format']} "
-        f"in {time.time() - t0:.1f} s")
-    msmp, phmp = time_plan(pmp, args.steps, args.warmup)
-    ymp = result(pmp)
-    log(f"valr: {msmp:.3f} ms/MVM  phases {phmp.round(4).tolist()} ms")
-    pmp.close()
+    if args.precision in ("both", "valr"):
+        t0 = time.time()
+        pmp = Plan.from_hmatrix(hm, precision="valr", eps=args.eps)
+        imp = pmp.info()
+        log(f"valr plan: {imp['factor_bytes'] / 1e9:.2f} GB cols/format {imp['cols_by_format']} "
+            f"in {time.time() - t0:.1f} s")
+        msmp, phmp = time_plan(pmp, args.steps, args.warmup)
+        ymp = result(pmp)
+        log(f"valr: {msmp:.3f} ms/MVM  phases {phmp.round(4).tolist()} ms")
+        pmp.close()
+
+    if args.precision != "both":
+        log("single-precision profiling run: no JSON line")
+        return 0
 
     err_gpu = float(np.linalg.norm(ymp - y64) / np.linalg.norm(y64))
     err_oracle = None
@@ -178,6 +210,7 @@ def main() -> int:
         "hbm_gbs_fp64": gbs_64,
         "hbm_frac_mixed": gbs_mp / HBM_PEAK_GBS,
         "hbm_frac_fp64": gbs_64 / HBM_PEAK_GBS,
+        "device_copy_gbs": copy_gbs,
         "bytes_per_mvm_mixed": mvm_bytes,
         "bytes_per_mvm_fp64": i64["datasize"],
         "phases_ms_mixed": phmp.tolist(),

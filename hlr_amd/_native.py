@@ -23,6 +23,7 @@ OP_NORMAL, OP_TRANSPOSED, OP_ADJOINT, OP_CONJUGATE = 0, 1, 2, 3
 BLOCK_DENSE, BLOCK_LOWRANK, BLOCK_LOWRANK_SV = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_VALR = 0, 1, 2
 F64, F32 = 0, 1
+FMT_F64, FMT_F32, FMT_BF16, FMT_FP8 = 0, 1, 2, 3
 
 KERNEL_LOG, KERNEL_LOG_GALERKIN, KERNEL_MATERN, KERNEL_INVR = 0, 1, 2, 3
 GEOM_LINE, GEOM_GRID1D, GEOM_CUBE, GEOM_SPHERE = 0, 1, 2, 3
@@ -38,7 +39,7 @@ class HmvmBlock(C.Structure):
 
 class HmvmPlanOpts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("dense_prec", C.c_int32), ("eps", C.c_double),
-                ("check_finite", C.c_int32), ("reserved", C.c_int32)]
+                ("check_finite", C.c_int32), ("force_format", C.c_int32)]
 
 
 class HmvmPlanInfo(C.Structure):

@@ -2,16 +2,15 @@
 //
 // Forward y += a A x (arith/mulvec.hh:27-64 semantics, leaves as in
 // matrix/lrmatrix.hh:625-679 and matrix/dense_matrix.hh:459-481):
-//   k_vfwd    P[e, q] = sum_{i in q} V(i, e) x_i     one wave per <= 64 column
-//                                                    groups of a column panel,
-//                                                    lane = column group
-//   k_tsum    t_e = a S_e sum_q P[e, q]              one thread per t entry
+//   k_vfwd    P[e, q] = sum_{i in q} V(i, e) x_i     one wave per pack of <= 64
+//                                                    column groups (lane = group)
+//   k_psum*   t_e = a S_e sum_q P[e, q]              two-stage segmented sums
 //   k_ufwd    y_p += sum_c R_p(:, c) z_c             one workgroup per row
 //             z_c = a x_j (dense) | t_e (low-rank)   panel, lane = row
 // Transposed y += a A^T x (arith/blas.hh:910-948):
 //   k_rtrn    Q[c] = sum_{i in p} R_p(i, c) x_i      wave per chunk, lane =
 //                                                    row, butterfly reduction
-//   k_tsum    s_e = a S_e sum_p Q[e, p]
+//   k_psum*   s_e = a S_e sum_p Q[e, p]
 //   k_vtrn    y_q += V_q s + a sum Q_dense           workgroup per column panel
 // No atomics: every output element has exactly one writer and a fixed
 // summation order, so results are bitwise reproducible run to run.
@@ -33,102 +32,149 @@ using namespace hmvm;
 
 namespace {
 
-constexpr int kWaves = 4;            // waves per workgroup
 constexpr int kThreads = 64 * kWaves;
+constexpr int64_t kWholeZBytes = 48 * 1024;   // largest panel z block staged whole in LDS
+
+template <typename T> struct IsFp8 { static constexpr bool value = false; };
+template <> struct IsFp8<uint8_t> { static constexpr bool value = true; };
 
 // ---------------------------------------------------------------------------
-// forward: V pass
+// forward: V pass, one wave per pack of consecutive segments (lane = group)
 // ---------------------------------------------------------------------------
 
 template <typename T, typename XT>
-__global__ __launch_bounds__(kThreads) void k_vfwd(DevPlan P, const VItem* __restrict__ items, int64_t nitems,
-                                                   const XT* __restrict__ x, double* __restrict__ Pb) {
+__device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp, const XT* __restrict__ x,
+                                 double* __restrict__ Pb, double* xsraw, int lane) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
-    __shared__ A xs[kWaves][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t item = int64_t(blockIdx.x) * kWaves + wave;
-    if (item >= nitems) return;
-    const VItem it = items[item];
-    const ColPanel C = P.cp[it.q];
-    const int m = C.m;
-    const int f = it.fmt;
-    if (lane < m) xs[wave][lane] = A(x[C.col0 + lane]);
+    const VPack& pk = *pkp;     // read through the pointer: the arrays stay out of scratch
+    A* xs = reinterpret_cast<A*>(xsraw);            // [kPackSegs][64]
+    const int nseg = pk.nseg;
+    // stage the x slices of all segments: 8 lanes per segment row block
+    for (int idx = lane; idx < nseg * 64; idx += 64) {
+        const int j = idx >> 6, i = idx & 63;
+        if (i < pk.m[j]) xs[idx] = A(x[pk.x0[j] + i]);
+    }
     wave_sync();
-    const int64_t kp = C.kpad[f];
-    int gp = 1, lg = 0;
-    while (gp < it.ng) { gp <<= 1; ++lg; }
-    const int g = lane & (gp - 1), ph = lane >> lg, nph = 64 >> lg;
-    const bool active = g < it.ng;
+    if (lane >= pk.gofs[nseg]) return;
+    int j = 0;
+    while (j + 1 < nseg && lane >= pk.gofs[j + 1]) ++j;
+    const int g = lane - pk.gofs[j];
+    const int m = pk.m[j];
+    const int64_t ld = int64_t(pk.ng[j]) * W;
+    const T* base = static_cast<const T*>(P.arena[E::F]) + pk.data_off[j] + int64_t(g) * W;
+    const A* xj = xs + j * 64;
     A acc[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) acc[w] = A(0);
-    if (active) {
-        const T* base = static_cast<const T*>(P.arena[f]) + C.off[f] + int64_t(it.g0 + g) * W;
-#pragma unroll 4
-        for (int i = ph; i < m; i += nph) {
-            A v[W];
-            E::load(base + int64_t(i) * kp, v);
-            const A xi = xs[wave][i];
+    constexpr int kUnroll = (W >= 8) ? 4 : 8;
+#pragma unroll kUnroll
+    for (int i = 0; i < m; ++i) {
+        A v[W];
+        E::load(base + int64_t(i) * ld, v);
+        const A xi = xj[i];
 #pragma unroll
-            for (int w = 0; w < W; ++w) acc[w] += v[w] * xi;
-        }
+        for (int w = 0; w < W; ++w) acc[w] += v[w] * xi;
     }
-    for (int s = gp; s < 64; s <<= 1) {
+    const int64_t vc0 = int64_t(pk.vc0[j]) + int64_t(g) * W;
 #pragma unroll
-        for (int w = 0; w < W; ++w) acc[w] += __shfl_xor(acc[w], s, 64);
-    }
-    if (active && ph == 0) {
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const int64_t vc = C.vcol0[f] + int64_t(it.g0 + g) * W + w;
-            const int32_t d = P.vdest[vc];
-            if (d >= 0) Pb[d] = double(acc[w]) * (P.vscale ? double(P.vscale[vc]) : 1.0);
-        }
+    for (int w = 0; w < W; ++w) {
+        const int32_t d = P.vdest[vc0 + w];
+        double v = double(acc[w]);
+        if (IsFp8<T>::value) v *= double(P.vscale[vc0 + w]);
+        if (d >= 0) Pb[d] = v;
     }
 }
 
-// t_e = alpha * S_e * sum_r P[off_e + r]
-__global__ __launch_bounds__(256) void k_tsum(const int32_t* __restrict__ off, const int32_t* __restrict__ cnt,
-                                              const double* __restrict__ tmul, const double* __restrict__ Pb,
-                                              double* __restrict__ t, int64_t ne, double alpha) {
+template <typename XT>
+__global__ __launch_bounds__(kThreads) void k_vfwd(DevPlan P, const XT* __restrict__ x, double* __restrict__ Pb) {
+    __shared__ double xs[kWaves][kPackSegs * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t pi = int64_t(blockIdx.x) * kWaves + wave;
+    if (pi >= P.nvpacks) return;
+    const VPack* pk = P.vpacks + pi;
+    switch (pk->fmt) {
+    case F64: vpack_fwd<double>(P, pk, x, Pb, xs[wave], lane); break;
+    case F32: vpack_fwd<float>(P, pk, x, Pb, xs[wave], lane); break;
+    case BF16: vpack_fwd<uint16_t>(P, pk, x, Pb, xs[wave], lane); break;
+    default: vpack_fwd<uint8_t>(P, pk, x, Pb, xs[wave], lane); break;
+    }
+}
+
+// stage 1: tmp[s] = sum of kSegLen consecutive partials of entry se[s]
+__global__ __launch_bounds__(256) void k_psum1(const int32_t* __restrict__ se, const int32_t* __restrict__ sr0,
+                                               const int32_t* __restrict__ off, const int32_t* __restrict__ cnt,
+                                               const int32_t* __restrict__ str, const double* __restrict__ Pb,
+                                               double* __restrict__ tmp, int64_t ns) {
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int32_t e = se[s], r0 = sr0[s];
+    const int32_t o = off[e], k = str[e];
+    const int32_t r1 = min(r0 + kSegLen, cnt[e]);
+    double v[kSegLen];
+#pragma unroll
+    for (int r = 0; r < kSegLen; ++r) v[r] = (r0 + r < r1) ? Pb[o + int64_t(r0 + r) * k] : 0.0;
+    double a = 0;
+#pragma unroll
+    for (int r = 0; r < kSegLen; ++r) a += v[r];
+    tmp[s] = a;
+}
+
+// stage 2: t_e = alpha * S_e * sum_j tmp[pos0_e + j * stride_e]
+__global__ __launch_bounds__(256) void k_psum2(const int32_t* __restrict__ pos0, const int32_t* __restrict__ nseg,
+                                               const int32_t* __restrict__ str, const double* __restrict__ tmul,
+                                               const double* __restrict__ tmp, double* __restrict__ t, int64_t ne,
+                                               double alpha) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= ne) return;
-    const int32_t o = off[e], c = cnt[e];
-    double s = 0;
-    for (int32_t r = 0; r < c; ++r) s += Pb[o + r];
-    t[e] = alpha * tmul[e] * s;
+    const int32_t p = pos0[e], n = nseg[e], k = str[e];
+    double a = 0;
+    for (int32_t j = 0; j < n; ++j) a += tmp[p + int64_t(j) * k];
+    t[e] = alpha * tmul[e] * a;
 }
 
 // ---------------------------------------------------------------------------
-// forward: R pass (dense rows and U rows of a row panel)
+// forward: R pass (dense rows and U rows of a row panel), one workgroup per
+// row panel; a chunk's z values are staged in LDS by the whole workgroup and
+// each wave streams a contiguous quarter of the chunk's groups
 // ---------------------------------------------------------------------------
 
 template <typename T, typename XT>
-__device__ inline double chunk_fwd(const DevPlan& P, const RChunk& C, int m, double alpha,
-                                   const XT* __restrict__ x, const double* __restrict__ t, void* zbuf, int lane) {
+__device__ inline void chunk_stage(const DevPlan& P, const RChunk& C, double alpha, const XT* __restrict__ x,
+                                   const double* __restrict__ t, void* zbuf, int tid) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
     A* zs = static_cast<A*>(zbuf);
     const int ncol = C.ngroups * W;
-    wave_sync();
-    for (int j = lane; j < ncol; j += 64) {
+    for (int j = tid; j < ncol; j += kThreads) {
         const int32_t s = P.colsrc[C.col0 + j];
         double z = (s >= 0) ? alpha * double(x[s]) : (s == kPadCol ? 0.0 : t[~s]);
-        if (P.zscale) z *= double(P.zscale[C.col0 + j]);
+        if (IsFp8<T>::value) z *= double(P.zscale[C.col0 + j]);
         zs[j] = A(z);
     }
-    wave_sync();
+}
+
+template <typename T>
+__device__ inline double chunk_compute(const DevPlan& P, const RChunk& C, int m, const void* zbuf, int wave,
+                                       int lane) {
+    using E = Elem<T>;
+    using A = typename E::Acc;
+    constexpr int W = E::W;
+    const A* zs = static_cast<const A*>(zbuf);
+    const int per = (C.ngroups + kWaves - 1) / kWaves;
+    const int gb = wave * per;
+    const int ge = min(gb + per, int(C.ngroups));
     A acc[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) acc[w] = A(0);
     if (lane < m) {
-        const T* base = static_cast<const T*>(P.arena[C.fmt]) + C.data_off + int64_t(lane) * W;
+        const T* base = static_cast<const T*>(P.arena[E::F]) + C.data_off + int64_t(lane) * W;
         const int64_t gs = int64_t(m) * W;
-#pragma unroll 8
-        for (int g = 0; g < C.ngroups; ++g) {
+        constexpr int kUnroll = (W >= 8) ? 4 : 8;
+#pragma unroll kUnroll
+        for (int g = gb; g < ge; ++g) {
             A v[W];
             E::load(base + g * gs, v);
 #pragma unroll
@@ -141,21 +187,55 @@ __device__ inline double chunk_fwd(const DevPlan& P, const RChunk& C, int m, dou
     return r;
 }
 
-template <typename XT, typename YT>
+// kWhole: the panel's z values fit in dynamic LDS and are staged once;
+// otherwise each chunk is staged into a fixed buffer between barriers.
+template <typename XT, typename YT, bool kWhole>
 __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, const XT* __restrict__ x,
                                                    const double* __restrict__ t, YT* __restrict__ y) {
-    __shared__ __align__(16) float zbuf[kWaves][1024];
+    extern __shared__ __align__(16) unsigned char zdyn[];
     __shared__ double red[kWaves][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const RowPanel R = P.rp[blockIdx.x];
     double acc = 0;
-    for (int ci = wave; ci < R.nchunks; ci += kWaves) {
-        const RChunk C = P.rchunks[R.chunk0 + ci];
-        switch (C.fmt) {
-        case F64: acc += chunk_fwd<double>(P, C, R.m, alpha, x, t, zbuf[wave], lane); break;
-        case F32: acc += chunk_fwd<float>(P, C, R.m, alpha, x, t, zbuf[wave], lane); break;
-        case BF16: acc += chunk_fwd<uint16_t>(P, C, R.m, alpha, x, t, zbuf[wave], lane); break;
-        default: acc += chunk_fwd<uint8_t>(P, C, R.m, alpha, x, t, zbuf[wave], lane); break;
+    if (kWhole) {
+        for (int ci = 0; ci < R.nchunks; ++ci) {
+            const RChunk C = P.rchunks[R.chunk0 + ci];
+            void* zb = zdyn + C.zoff;
+            switch (C.fmt) {
+            case F64: chunk_stage<double>(P, C, alpha, x, t, zb, tid); break;
+            case F32: chunk_stage<float>(P, C, alpha, x, t, zb, tid); break;
+            case BF16: chunk_stage<uint16_t>(P, C, alpha, x, t, zb, tid); break;
+            default: chunk_stage<uint8_t>(P, C, alpha, x, t, zb, tid); break;
+            }
+        }
+        __syncthreads();
+        for (int ci = 0; ci < R.nchunks; ++ci) {
+            const RChunk C = P.rchunks[R.chunk0 + ci];
+            const void* zb = zdyn + C.zoff;
+            switch (C.fmt) {
+            case F64: acc += chunk_compute<double>(P, C, R.m, zb, wave, lane); break;
+            case F32: acc += chunk_compute<float>(P, C, R.m, zb, wave, lane); break;
+            case BF16: acc += chunk_compute<uint16_t>(P, C, R.m, zb, wave, lane); break;
+            default: acc += chunk_compute<uint8_t>(P, C, R.m, zb, wave, lane); break;
+            }
+        }
+    } else {
+        for (int ci = 0; ci < R.nchunks; ++ci) {
+            const RChunk C = P.rchunks[R.chunk0 + ci];
+            __syncthreads();
+            switch (C.fmt) {
+            case F64: chunk_stage<double>(P, C, alpha, x, t, zdyn, tid); break;
+            case F32: chunk_stage<float>(P, C, alpha, x, t, zdyn, tid); break;
+            case BF16: chunk_stage<uint16_t>(P, C, alpha, x, t, zdyn, tid); break;
+            default: chunk_stage<uint8_t>(P, C, alpha, x, t, zdyn, tid); break;
+            }
+            __syncthreads();
+            switch (C.fmt) {
+            case F64: acc += chunk_compute<double>(P, C, R.m, zdyn, wave, lane); break;
+            case F32: acc += chunk_compute<float>(P, C, R.m, zdyn, wave, lane); break;
+            case BF16: acc += chunk_compute<uint16_t>(P, C, R.m, zdyn, wave, lane); break;
+            default: acc += chunk_compute<uint8_t>(P, C, R.m, zdyn, wave, lane); break;
+            }
         }
     }
     red[wave][lane] = acc;
@@ -167,7 +247,7 @@ __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, cons
 }
 
 // ---------------------------------------------------------------------------
-// transposed: R pass with column reduction
+// transposed: R pass with column reduction, one wave per chunk
 // ---------------------------------------------------------------------------
 
 template <typename T, typename XT>
@@ -184,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __r
     const RowPanel R = P.rp[C.panel];
     const int m = R.m;
     const A xi = (lane < m) ? A(x[R.row0 + lane]) : A(0);
-    const T* base = static_cast<const T*>(P.arena[C.fmt]) + C.data_off + int64_t(lane) * W;
+    const T* base = static_cast<const T*>(P.arena[E::F]) + C.data_off + int64_t(lane) * W;
     const int64_t gs = int64_t(m) * W;
     for (int g0 = 0; g0 < C.ngroups; g0 += SG) {
         A v[64];
@@ -206,37 +286,41 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __r
         if (col < C.ngroups * W) {
             const int64_t gc = C.col0 + col;
             const int32_t d = P.rdest[gc];
-            if (d >= 0) Q[d] = double(s) * (P.zscale ? double(P.zscale[gc]) : 1.0);
+            double sv = double(s);
+            if (IsFp8<T>::value) sv *= double(P.zscale[gc]);
+            if (d >= 0) Q[d] = sv;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// transposed: V pass, y_q += V_q s  (+ dense partials)
+// transposed: V pass, y_q += V_q s  (+ dense partials), workgroup per panel
 // ---------------------------------------------------------------------------
 
 template <typename T>
-__device__ inline void chunk_vtrn(const DevPlan& P, const ColPanel& C, int f, int g0, int ng,
-                                  const double* __restrict__ s, double (&acc)[64], int lane) {
+__device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VItem& it,
+                                 const double* __restrict__ s, double (&acc)[64], int lane) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
-    const int g = g0 + lane;
-    if (g >= ng) return;
+    if (lane >= it.ng) return;
+    const int g = lane;
     double sv[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        const int64_t vc = C.vcol0[f] + int64_t(g) * W + w;
+        const int64_t vc = C.vcol0[E::F] + int64_t(it.g0 + g) * W + w;
         const int32_t e = P.vsrc[vc];
-        sv[w] = (e >= 0) ? s[e] * (P.vscale ? double(P.vscale[vc]) : 1.0) : 0.0;
+        double v = (e >= 0) ? s[e] : 0.0;
+        if (IsFp8<T>::value) v *= double(P.vscale[vc]);
+        sv[w] = v;
     }
-    const int64_t kp = C.kpad[f];
-    const T* base = static_cast<const T*>(P.arena[f]) + C.off[f] + int64_t(g) * W;
+    const int64_t ld = int64_t(it.ng) * W;
+    const T* base = static_cast<const T*>(P.arena[E::F]) + it.data_off + int64_t(g) * W;
 #pragma unroll
     for (int i = 0; i < 64; ++i) {
         if (i < C.m) {
             A v[W];
-            E::load(base + int64_t(i) * kp, v);
+            E::load(base + int64_t(i) * ld, v);
             double d = 0;
 #pragma unroll
             for (int w = 0; w < W; ++w) d += double(v[w]) * sv[w];
@@ -254,18 +338,13 @@ __global__ __launch_bounds__(kThreads) void k_vtrn(DevPlan P, double alpha, cons
     double acc[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = 0;
-    int ck = 0;
-    for (int f = 0; f < NFMT; ++f) {
-        const int W = fmt_width(f);
-        const int ng = C.kpad[f] / W;
-        for (int g0 = 0; g0 < ng; g0 += 64, ++ck) {
-            if ((ck % kWaves) != wave) continue;
-            switch (f) {
-            case F64: chunk_vtrn<double>(P, C, f, g0, ng, s, acc, lane); break;
-            case F32: chunk_vtrn<float>(P, C, f, g0, ng, s, acc, lane); break;
-            case BF16: chunk_vtrn<uint16_t>(P, C, f, g0, ng, s, acc, lane); break;
-            default: chunk_vtrn<uint8_t>(P, C, f, g0, ng, s, acc, lane); break;
-            }
+    for (int j = wave; j < C.nitems; j += kWaves) {
+        const VItem it = P.vitems[C.item0 + j];
+        switch (it.fmt) {
+        case F64: item_vtrn<double>(P, C, it, s, acc, lane); break;
+        case F32: item_vtrn<float>(P, C, it, s, acc, lane); break;
+        case BF16: item_vtrn<uint16_t>(P, C, it, s, acc, lane); break;
+        default: item_vtrn<uint8_t>(P, C, it, s, acc, lane); break;
         }
     }
     const double r = butterfly64(acc, lane);
@@ -310,31 +389,29 @@ __global__ __launch_bounds__(kThreads) void k_grad_r(DevPlan P, const int32_t* _
     }
 }
 
-// dV(i, c) = x_i * s_e
-template <typename T, typename XT>
-__global__ __launch_bounds__(kThreads) void k_grad_v(DevPlan P, const VItem* __restrict__ items, int64_t nitems,
-                                                     const XT* __restrict__ x, const double* __restrict__ s,
-                                                     double* __restrict__ gV) {
-    constexpr int W = Elem<T>::W;
+struct GradPtrs {
+    double* p[NFMT];
+};
+
+// dV(i, c) = x_i * s_e, one wave per item, lane = group
+template <typename XT>
+__global__ __launch_bounds__(kThreads) void k_grad_v(DevPlan P, const XT* __restrict__ x,
+                                                     const double* __restrict__ s, GradPtrs gA) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t item = int64_t(blockIdx.x) * kWaves + wave;
-    if (item >= nitems) return;
-    const VItem it = items[item];
+    if (item >= P.nvitems) return;
+    const VItem it = P.vitems[item];
+    double* gV = gA.p[it.fmt];
+    if (!gV || lane >= it.ng) return;
     const ColPanel C = P.cp[it.q];
-    const int f = it.fmt;
-    const int64_t kp = C.kpad[f];
-    if (lane >= it.ng) return;
-    const int g = it.g0 + lane;
-    double sv[W];
-#pragma unroll
+    const int W = fmt_width(it.fmt);
+    const int64_t ld = int64_t(it.ng) * W;
     for (int w = 0; w < W; ++w) {
-        const int32_t e = P.vsrc[C.vcol0[f] + int64_t(g) * W + w];
-        sv[w] = (e >= 0) ? s[e] : 0.0;
-    }
-    for (int i = 0; i < C.m; ++i) {
-        const double xi = double(x[C.col0 + i]);
-#pragma unroll
-        for (int w = 0; w < W; ++w) gV[C.off[f] + int64_t(i) * kp + int64_t(g) * W + w] = xi * sv[w];
+        const int64_t vc = C.vcol0[it.fmt] + int64_t(it.g0 + lane) * W + w;
+        const int32_t e = P.vsrc[vc];
+        const double sv = (e >= 0) ? s[e] : 0.0;
+        for (int i = 0; i < C.m; ++i)
+            gV[it.data_off + int64_t(i) * ld + int64_t(lane) * W + w] = double(x[C.col0 + i]) * sv;
     }
 }
 
@@ -350,9 +427,7 @@ struct hmvm_plan {
     DevPlan dp{};
     std::vector<void*> allocs;
     void* arena_dev[NFMT] = {nullptr, nullptr, nullptr, nullptr};
-    // work lists on the device, grouped by format
-    VItem* vitems_dev = nullptr;
-    int64_t vitem_start[NFMT + 1] = {0, 0, 0, 0, 0};
+    // R chunk ids grouped by format (transposed and gradient launches)
     int32_t* rchunk_ids_dev = nullptr;
     int64_t rchunk_start[NFMT + 1] = {0, 0, 0, 0, 0};
 };
@@ -408,18 +483,18 @@ int upload_plan(hmvm_plan* p) {
     UP(qt_cnt, L.qt_cnt);
     UP(qd_off, L.qd_off);
     UP(qd_cnt, L.qd_cnt);
+    UP(tstride, L.tstride);
+    UP(vitems, L.vitems);
+    UP(vpacks, L.vpacks);
+    UP(ps_e, L.ps_e);
+    UP(ps_r0, L.ps_r0);
+    UP(ps_pos0, L.ps_pos0);
+    UP(ps_nseg, L.ps_nseg);
+    UP(qs_e, L.qs_e);
+    UP(qs_r0, L.qs_r0);
+    UP(qs_pos0, L.qs_pos0);
+    UP(qs_nseg, L.qs_nseg);
 #undef UP
-    // work lists grouped by format
-    std::vector<VItem> vi;
-    for (int f = 0; f < NFMT; ++f) {
-        p->vitem_start[f] = int64_t(vi.size());
-        for (const VItem& it : L.vitems) if (it.fmt == f) vi.push_back(it);
-    }
-    p->vitem_start[NFMT] = int64_t(vi.size());
-    const VItem* vd = nullptr;
-    if ((rc = upload(p, vi, &vd)) != HMVM_OK) return rc;
-    p->vitems_dev = const_cast<VItem*>(vd);
-    D.vitems = vd;
     std::vector<int32_t> ids;
     for (int f = 0; f < NFMT; ++f) {
         p->rchunk_start[f] = int64_t(ids.size());
@@ -438,7 +513,10 @@ int upload_plan(hmvm_plan* p) {
     for (int f = 0; f < NFMT; ++f) { D.arena[f] = a[f]; p->arena_dev[f] = const_cast<void*>(a[f]); }
     D.nrp = int64_t(L.rp.size());
     D.ncp = int64_t(L.cp.size());
-    D.nvitems = int64_t(vi.size());
+    D.nvitems = int64_t(L.vitems.size());
+    D.nvpacks = int64_t(L.vpacks.size());
+    D.nps = int64_t(L.ps_e.size());
+    D.nqs = int64_t(L.qs_e.size());
     D.nrchunks = int64_t(L.rchunks.size());
     D.ntent = L.ntent;
     D.psize = L.psize;
@@ -455,11 +533,28 @@ int upload_plan(hmvm_plan* p) {
 
 inline unsigned blocks_for(int64_t n, int per) { return unsigned((n + per - 1) / per); }
 
+inline int64_t pad32(int64_t v) { return ((v + 31) / 32) * 32; }
+
+// workspace: partials | stage-1 segment sums | t (or s) + 32 spare
 int64_t ws_doubles(const hmvm_plan* p, int op) {
     const bool trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
     const HostLayout& L = p->L;
     const int64_t a = trans ? L.qsize : L.psize;
-    return ((a + 31) / 32) * 32 + ((L.ntent + 31) / 32) * 32 + 32;
+    const int64_t sg = trans ? int64_t(L.qs_e.size()) : int64_t(L.ps_e.size());
+    return pad32(a) + pad32(sg) + pad32(L.ntent) + 32;
+}
+
+// t = alpha S sum(partials) in two stages
+void launch_psum(const DevPlan& D, bool trans, const double* part, double* tmp, double* t, double alpha,
+                 hipStream_t st) {
+    const int64_t ns = trans ? D.nqs : D.nps;
+    if (D.ntent == 0) return;
+    if (ns > 0)
+        k_psum1<<<blocks_for(ns, 256), 256, 0, st>>>(trans ? D.qs_e : D.ps_e, trans ? D.qs_r0 : D.ps_r0,
+                                                     trans ? D.qt_off : D.pt_off, trans ? D.qt_cnt : D.pt_cnt,
+                                                     D.tstride, part, tmp, ns);
+    k_psum2<<<blocks_for(D.ntent, 256), 256, 0, st>>>(trans ? D.qs_pos0 : D.ps_pos0, trans ? D.qs_nseg : D.ps_nseg,
+                                                      D.tstride, D.tmul, tmp, t, D.ntent, alpha);
 }
 
 template <typename XT, typename YT>
@@ -469,27 +564,24 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
     const HostLayout& L = p->L;
     const bool trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
     const int64_t abuf = trans ? L.qsize : L.psize;
+    const int64_t nseg = trans ? D.nqs : D.nps;
     double* part = ws;
-    double* t = ws + ((abuf + 31) / 32) * 32;
+    double* tmp = ws + pad32(abuf);
+    double* t = tmp + pad32(nseg);
     if (ev) (void)hipEventRecord(ev[0], st);
     if (!trans) {
-        for (int f = 0; f < NFMT; ++f) {
-            const int64_t n = p->vitem_start[f + 1] - p->vitem_start[f];
-            if (n == 0) continue;
-            const VItem* it = p->vitems_dev + p->vitem_start[f];
-            const unsigned nb = blocks_for(n, kWaves);
-            switch (f) {
-            case F64: k_vfwd<double, XT><<<nb, kThreads, 0, st>>>(D, it, n, x, part); break;
-            case F32: k_vfwd<float, XT><<<nb, kThreads, 0, st>>>(D, it, n, x, part); break;
-            case BF16: k_vfwd<uint16_t, XT><<<nb, kThreads, 0, st>>>(D, it, n, x, part); break;
-            default: k_vfwd<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, it, n, x, part); break;
-            }
-        }
+        if (D.nvpacks > 0) k_vfwd<XT><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, x, part);
         if (ev) (void)hipEventRecord(ev[1], st);
-        if (L.ntent > 0)
-            k_tsum<<<blocks_for(L.ntent, 256), 256, 0, st>>>(D.pt_off, D.pt_cnt, D.tmul, part, t, L.ntent, alpha);
+        launch_psum(D, false, part, tmp, t, alpha, st);
         if (ev) (void)hipEventRecord(ev[2], st);
-        if (D.nrp > 0) k_ufwd<XT, YT><<<unsigned(D.nrp), kThreads, 0, st>>>(D, alpha, x, t, y);
+        if (D.nrp > 0) {
+            // chunk staging needs kChunkGroups * 16 floats (FP8) or * 2 doubles (FP64): 16 KB
+            if (p->L.max_zbytes <= kWholeZBytes)
+                k_ufwd<XT, YT, true><<<unsigned(D.nrp), kThreads, size_t(p->L.max_zbytes), st>>>(D, alpha, x, t, y);
+            else
+                k_ufwd<XT, YT, false><<<unsigned(D.nrp), kThreads, size_t(kChunkGroups * 16 * 4), st>>>(D, alpha, x, t,
+                                                                                                          y);
+        }
         if (ev) (void)hipEventRecord(ev[3], st);
     } else {
         for (int f = 0; f < NFMT; ++f) {
@@ -505,8 +597,7 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
             }
         }
         if (ev) (void)hipEventRecord(ev[1], st);
-        if (L.ntent > 0)
-            k_tsum<<<blocks_for(L.ntent, 256), 256, 0, st>>>(D.qt_off, D.qt_cnt, D.tmul, part, t, L.ntent, alpha);
+        launch_psum(D, true, part, tmp, t, alpha, st);
         if (ev) (void)hipEventRecord(ev[2], st);
         if (D.ncp > 0) k_vtrn<YT><<<unsigned(D.ncp), kThreads, 0, st>>>(D, alpha, t, part, y);
         if (ev) (void)hipEventRecord(ev[3], st);
@@ -723,31 +814,22 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
     hipStream_t st = static_cast<hipStream_t>(stream);
     const HostLayout& L = p->L;
     const DevPlan& D = p->dp;
-    const int64_t np = ((L.psize + 31) / 32) * 32, nq = ((L.qsize + 31) / 32) * 32, ne = ((L.ntent + 31) / 32) * 32 + 32;
-    const int64_t bytes = (np + nq + 2 * ne) * int64_t(sizeof(double));
+    const int64_t np = pad32(L.psize), nq = pad32(L.qsize), ne = pad32(L.ntent) + 32;
+    const int64_t nsg = pad32(std::max<int64_t>(D.nps, D.nqs));
+    const int64_t bytes = (np + nq + 2 * ne + nsg) * int64_t(sizeof(double));
     void* ws = nullptr;
     if (hipMallocAsync(&ws, size_t(bytes), st) != hipSuccess) return HMVM_ERR_ALLOC;
     double* Pb = static_cast<double*>(ws);
     double* Qb = Pb + np;
     double* t = Qb + nq;
     double* s = t + ne;
+    double* tmp = s + ne;
     auto run = [&](auto xt_tag) -> int {
         using XT = decltype(xt_tag);
         const XT* xx = static_cast<const XT*>(x);
         const XT* yb = static_cast<const XT*>(ybar);
         // t = S V^T x, s = S U^T ybar (alpha = 1)
-        for (int f = 0; f < NFMT; ++f) {
-            const int64_t n = p->vitem_start[f + 1] - p->vitem_start[f];
-            if (n == 0) continue;
-            const VItem* it = p->vitems_dev + p->vitem_start[f];
-            const unsigned nb = blocks_for(n, kWaves);
-            switch (f) {
-            case F64: k_vfwd<double, XT><<<nb, kThreads, 0, st>>>(D, it, n, xx, Pb); break;
-            case F32: k_vfwd<float, XT><<<nb, kThreads, 0, st>>>(D, it, n, xx, Pb); break;
-            case BF16: k_vfwd<uint16_t, XT><<<nb, kThreads, 0, st>>>(D, it, n, xx, Pb); break;
-            default: k_vfwd<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, it, n, xx, Pb); break;
-            }
-        }
+        if (D.nvpacks > 0) k_vfwd<XT><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, xx, Pb);
         for (int f = 0; f < NFMT; ++f) {
             const int64_t n = p->rchunk_start[f + 1] - p->rchunk_start[f];
             if (n == 0) continue;
@@ -760,10 +842,8 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
             default: k_rtrn<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb); break;
             }
         }
-        if (L.ntent > 0) {
-            k_tsum<<<blocks_for(L.ntent, 256), 256, 0, st>>>(D.pt_off, D.pt_cnt, D.tmul, Pb, t, L.ntent, 1.0);
-            k_tsum<<<blocks_for(L.ntent, 256), 256, 0, st>>>(D.qt_off, D.qt_cnt, D.tmul, Qb, s, L.ntent, 1.0);
-        }
+        launch_psum(D, false, Pb, tmp, t, 1.0, st);
+        launch_psum(D, true, Qb, tmp, s, 1.0, st);
         if (gx && D.ncp > 0) k_vtrn<XT><<<unsigned(D.ncp), kThreads, 0, st>>>(D, 1.0, s, Qb, static_cast<XT*>(gx));
         if (grad_arena) {
             for (int f = 0; f < NFMT; ++f) {
@@ -780,18 +860,10 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
                     default: k_grad_r<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, ids, nr, xx, yb, t, gA); break;
                     }
                 }
-                const int64_t nv = p->vitem_start[f + 1] - p->vitem_start[f];
-                if (nv > 0) {
-                    const VItem* it = p->vitems_dev + p->vitem_start[f];
-                    const unsigned nb = blocks_for(nv, kWaves);
-                    switch (f) {
-                    case F64: k_grad_v<double, XT><<<nb, kThreads, 0, st>>>(D, it, nv, xx, s, gA); break;
-                    case F32: k_grad_v<float, XT><<<nb, kThreads, 0, st>>>(D, it, nv, xx, s, gA); break;
-                    case BF16: k_grad_v<uint16_t, XT><<<nb, kThreads, 0, st>>>(D, it, nv, xx, s, gA); break;
-                    default: k_grad_v<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, it, nv, xx, s, gA); break;
-                    }
-                }
             }
+            GradPtrs gp;
+            for (int f = 0; f < NFMT; ++f) gp.p[f] = static_cast<double*>(grad_arena[f]);
+            if (D.nvitems > 0) k_grad_v<XT><<<blocks_for(D.nvitems, kWaves), kThreads, 0, st>>>(D, xx, s, gp);
         }
         return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
     };

@@ -33,6 +33,17 @@ struct DevPlan {
     const int32_t* pt_cnt;
     const int32_t* qt_off;
     const int32_t* qt_cnt;
+    const int32_t* tstride;
+    const VPack* vpacks;
+    const int32_t* ps_e;
+    const int32_t* ps_r0;
+    const int32_t* ps_pos0;
+    const int32_t* ps_nseg;
+    const int32_t* qs_e;
+    const int32_t* qs_r0;
+    const int32_t* qs_pos0;
+    const int32_t* qs_nseg;
+    int64_t nvpacks, nps, nqs;
     const int32_t* qd_off;
     const int32_t* qd_cnt;
     const void* arena[NFMT];
@@ -46,6 +57,7 @@ struct DevPlan {
 template <typename T> struct Elem;
 
 template <> struct Elem<double> {
+    static constexpr int F = F64;
     static constexpr int W = 2;
     using Acc = double;
     __device__ static inline void load(const double* p, double* out) {
@@ -55,6 +67,7 @@ template <> struct Elem<double> {
 };
 
 template <> struct Elem<float> {
+    static constexpr int F = F32;
     static constexpr int W = 4;
     using Acc = float;
     __device__ static inline void load(const float* p, float* out) {
@@ -64,6 +77,7 @@ template <> struct Elem<float> {
 };
 
 template <> struct Elem<uint16_t> {   // BF16
+    static constexpr int F = BF16;
     static constexpr int W = 8;
     using Acc = float;
     __device__ static inline void load(const uint16_t* p, float* out) {
@@ -78,6 +92,7 @@ template <> struct Elem<uint16_t> {   // BF16
 };
 
 template <> struct Elem<uint8_t> {    // FP8 E4M3 (OCP)
+    static constexpr int F = FP8;
     static constexpr int W = 16;
     using Acc = float;
     __device__ static inline void load(const uint8_t* p, float* out) {

@@ -3,12 +3,17 @@
 // The matrix is cut into row panels and column panels of at most 64 indices
 // (one wavefront).  Every leaf contributes
 //   - to each row panel it covers: its dense rows D[p,:] (width n) or its U
-//     rows U[p,:] (width k), stored column-group-interleaved ("R arena");
+//     rows U[p,:] (width k)                                     ("R arena");
 //   - to each column panel it covers (low-rank only): its V rows V[q,:]
-//     (width k), stored row-major ("V arena").
-// Columns inside a panel are grouped by storage format; a group of W columns
-// (W * sizeof(element) == 16 bytes) of one panel is m x W row-major, so one
-// lane (= one row) loads its W values with a single 16-byte load.
+//     (width k)                                                 ("V arena").
+// Columns inside a panel are grouped by storage format and cut into groups
+// of W columns, W * sizeof(element) == 16 bytes, so that a lane loads W
+// values of one row with a single 16-byte load.
+//   R: a chunk of up to kChunkGroups groups of one format is contiguous;
+//      group g of the chunk is an m x W row-major block (lane = row).
+//   V: an item of up to 64 groups of one format is contiguous; it is an
+//      m x (ng*W) row-major block (lane = group, rows streamed).
+// Every chunk and item is one contiguous stream of HBM.
 //
 // Forward  y += a A x :  P = V^T x per column panel (no reduction),
 //                        t = a * sum_q P,  y_p += R_p z  (z = a x | t).
@@ -28,7 +33,9 @@ namespace hmvm {
 enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, NFMT = 4 };
 
 constexpr int kPanel = 64;          // rows per panel = lanes per wave
-constexpr int kChunkGroups = 64;    // column groups per R chunk
+constexpr int kWaves = 4;           // waves per workgroup
+constexpr int kChunkGroups = 64 * kWaves;   // column groups per R chunk (one per lane of the workgroup)
+constexpr int kItemGroups = 64;     // column groups per V item (one per lane of a wave)
 constexpr int32_t kPadCol = INT32_MIN;
 
 HMVM_HD inline int fmt_bytes(int f) { return f == F64 ? 8 : f == F32 ? 4 : f == BF16 ? 2 : 1; }
@@ -39,6 +46,8 @@ struct RowPanel {
     int32_t m;          // rows (<= 64)
     int32_t nchunks;    // chunks of this panel
     int64_t chunk0;     // first chunk in the chunk table
+    int32_t zbytes;     // LDS bytes of all z values of the panel (decoded accumulation type)
+    int32_t pad_;
 };
 
 struct RChunk {         // up to kChunkGroups groups of one format of one row panel
@@ -47,23 +56,38 @@ struct RChunk {         // up to kChunkGroups groups of one format of one row pa
     int64_t data_off;   // element offset of group 0 in arena[fmt]; group g at data_off + g*m*W
     int64_t col0;       // first column in the per-R-column tables (ngroups*W columns)
     int32_t panel;      // row panel index
-    int32_t pad_;
+    int32_t zoff;       // byte offset of the chunk's z values in the panel's LDS block
 };
 
 struct ColPanel {
     int64_t col0;       // first global column
     int32_t m;          // rows of V in this panel (<= 64)
-    int32_t pad_;
-    int64_t off[NFMT];  // element offset of the m x kpad[f] row-major block in arena[f]
+    int32_t nitems;     // V items of this panel
+    int64_t item0;      // first item in the item table
     int32_t kpad[NFMT]; // padded width per format (multiple of W)
     int64_t vcol0[NFMT];// first column in the per-V-column tables
 };
 
-struct VItem {          // forward V work unit: <= 64 column groups of one format of one column panel
+constexpr int kPackSegs = 8;        // V segments per forward pack
+constexpr int kSegLen = 16;         // partial sums per thread in the first summation stage
+
+struct VItem {          // V segment: up to kItemGroups groups of one format of one column panel
     int32_t q;
     int32_t fmt;
-    int32_t g0;
+    int32_t g0;         // first group inside the panel's format block
     int32_t ng;
+    int64_t data_off;   // element offset of the m x (ng*W) row-major block in arena[fmt]
+};
+
+struct VPack {          // forward V work unit of one wave: consecutive segments of one format,
+    int32_t fmt;        // at most 64 groups in total (lane = group); everything a lane
+    int32_t nseg;       // needs is stored here, so the pack starts without load chains
+    int32_t gofs[kPackSegs + 1];   // prefix sums of the segments' group counts
+    int32_t m[kPackSegs];          // rows of each segment's column panel
+    int32_t ng[kPackSegs];         // groups of each segment
+    int32_t x0[kPackSegs];         // first x index (column panel start)
+    int32_t vc0[kPackSegs];        // per-V-column table index of each segment's group 0
+    int64_t data_off[kPackSegs];   // element offset of each segment's block
 };
 
 } // namespace hmvm

@@ -136,6 +136,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     if (nb < 0 || nrows < 0 || ncols < 0 || (nb > 0 && (!blocks || !arena))) return HMVM_ERR_INVALID_ARG;
     if (opts.precision < HMVM_PREC_FP64 || opts.precision > HMVM_PREC_VALR) return HMVM_ERR_INVALID_ARG;
     if (opts.precision == HMVM_PREC_VALR && !(opts.eps > 0)) return HMVM_ERR_INVALID_ARG;
+    if (opts.force_format < 0 || opts.force_format > NFMT) return HMVM_ERR_INVALID_ARG;
     if (nrows > INT32_MAX || ncols > INT32_MAX) return HMVM_ERR_UNSUPPORTED;
     L = HostLayout();
     L.nrows = nrows;
@@ -247,6 +248,8 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 valr_tolerances(b.rank, arena + b.off_s, opts.eps, tol.data());
                 for (int64_t l = 0; l < b.rank; ++l) L.tfmt[e0 + l] = int8_t(format_for_rate(eps_to_rate(tol[l])));
             }
+            if (opts.force_format > 0)
+                for (int64_t l = 0; l < b.rank; ++l) L.tfmt[e0 + l] = int8_t(opts.force_format - 1);
             for (int64_t l = 0; l < b.rank; ++l) L.cols_by_format[L.tfmt[e0 + l]]++;
         }
     }
@@ -274,20 +277,51 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         if (bad) return HMVM_ERR_OVERLAP;
     }
 
-    // ---- forward partial layout P (t-entry major) and transposed Q ----
+    // ---- forward partials P and transposed partials Q, block-major:
+    //      entry (panel index r, column l) of block b at base_b + r * k_b + l,
+    //      so the sums over r read consecutive l with unit stride ----
     L.pt_off.assign(ne, 0); L.pt_cnt.assign(ne, 0);
     L.qt_off.assign(ne, 0); L.qt_cnt.assign(ne, 0);
+    L.tstride.assign(ne, 0);
     {
         int64_t pos = 0, qpos = 0;
         for (int64_t i = 0; i < nb; ++i) {
             if (L.blk_t0[i] < 0) continue;
-            for (int64_t l = 0; l < blocks[i].rank; ++l) {
+            const int64_t k = blocks[i].rank;
+            for (int64_t l = 0; l < k; ++l) {
                 const int64_t e = L.blk_t0[i] + l;
-                L.pt_off[e] = int32_t(pos); L.pt_cnt[e] = int32_t(bq1[i] - bq0[i]); pos += bq1[i] - bq0[i];
-                L.qt_off[e] = int32_t(qpos); L.qt_cnt[e] = int32_t(bp1[i] - bp0[i]); qpos += bp1[i] - bp0[i];
+                L.pt_off[e] = int32_t(pos + l); L.pt_cnt[e] = int32_t(bq1[i] - bq0[i]);
+                L.qt_off[e] = int32_t(qpos + l); L.qt_cnt[e] = int32_t(bp1[i] - bp0[i]);
+                L.tstride[e] = int32_t(k);
             }
+            pos += k * (bq1[i] - bq0[i]);
+            qpos += k * (bp1[i] - bp0[i]);
         }
         L.psize = pos;
+        // summation segments, block-major (block, segment, column)
+        auto segs = [&](bool fwd, std::vector<int32_t>& se, std::vector<int32_t>& sr0, std::vector<int32_t>& pos0,
+                        std::vector<int32_t>& nsg) {
+            pos0.assign(ne, 0); nsg.assign(ne, 0);
+            int64_t sp = 0;
+            for (int64_t i = 0; i < nb; ++i) {
+                if (L.blk_t0[i] < 0) continue;
+                const int64_t k = blocks[i].rank;
+                const int64_t c = fwd ? (bq1[i] - bq0[i]) : (bp1[i] - bp0[i]);
+                const int64_t ns = (c + kSegLen - 1) / kSegLen;
+                for (int64_t l = 0; l < k; ++l) {
+                    pos0[L.blk_t0[i] + l] = int32_t(sp + l);
+                    nsg[L.blk_t0[i] + l] = int32_t(ns);
+                }
+                for (int64_t j = 0; j < ns; ++j)
+                    for (int64_t l = 0; l < k; ++l) {
+                        se.push_back(int32_t(L.blk_t0[i] + l));
+                        sr0.push_back(int32_t(j * kSegLen));
+                    }
+                sp += ns * k;
+            }
+        };
+        segs(true, L.ps_e, L.ps_r0, L.ps_pos0, L.ps_nseg);
+        segs(false, L.qs_e, L.qs_r0, L.qs_pos0, L.qs_nseg);
         // dense transposed partials: per global column, one per (dense block, row panel)
         L.qd_cnt.assign(ncols, 0);
         L.qd_off.assign(ncols, 0);
@@ -355,13 +389,23 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 c.data_off = arena_pos[f] + g0 * P.m * W;
                 c.col0 = colpos + g0 * W;
                 c.panel = int32_t(p);
-                c.pad_ = 0;
+                c.zoff = 0;
                 L.rchunks.push_back(c);
             }
             arena_pos[f] += kp * P.m;
             colpos += kp;
         }
         P.nchunks = int32_t(int64_t(L.rchunks.size()) - P.chunk0);
+        int64_t zb = 0;
+        for (int64_t c = P.chunk0; c < P.chunk0 + P.nchunks; ++c) {
+            RChunk& ch = L.rchunks[c];
+            const int ab = (ch.fmt == F64) ? 8 : 4;        // accumulation type size
+            zb = round_up(zb, 16);
+            ch.zoff = int32_t(zb);
+            zb += int64_t(ch.ngroups) * fmt_width(ch.fmt) * ab;
+        }
+        P.zbytes = int32_t(round_up(zb, 16));
+        L.max_zbytes = std::max<int64_t>(L.max_zbytes, P.zbytes);
     }
     const int64_t nrcols = colpos;
 
@@ -380,29 +424,59 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         ColPanel& C = L.cp[q];
         C.col0 = cpan[q].first;
         C.m = cpan[q].second;
-        C.pad_ = 0;
+        C.item0 = int64_t(L.vitems.size());
         for (int f = 0; f < NFMT; ++f) {
             const int W = fmt_width(f);
             const int64_t kp = round_up(ccols[q][f], W);
             C.kpad[f] = int32_t(kp);
-            C.off[f] = varena_pos[f];
             C.vcol0[f] = vpos;
-            varena_pos[f] += kp * C.m;
             vpos += kp;
             const int64_t ng = kp / W;
-            // work items of <= 64 groups, as even as possible
+            // items of <= kItemGroups groups, as even as possible, each one
+            // contiguous m x (ng * W) row-major block
             if (ng > 0) {
-                const int64_t nit = (ng + 63) / 64;
+                const int64_t nit = (ng + kItemGroups - 1) / kItemGroups;
                 const int64_t base = ng / nit, extra = ng % nit;
                 int64_t g = 0;
                 for (int64_t it = 0; it < nit; ++it) {
                     const int64_t cnt = base + (it < extra ? 1 : 0);
-                    L.vitems.push_back({int32_t(q), int32_t(f), int32_t(g), int32_t(cnt)});
+                    VItem v;
+                    v.q = int32_t(q); v.fmt = int32_t(f); v.g0 = int32_t(g); v.ng = int32_t(cnt);
+                    v.data_off = varena_pos[f];
+                    varena_pos[f] += int64_t(C.m) * cnt * W;
+                    L.vitems.push_back(v);
                     g += cnt;
                 }
             }
         }
+        C.nitems = int32_t(int64_t(L.vitems.size()) - C.item0);
         L.max_vitem_rows = std::max<int32_t>(L.max_vitem_rows, C.m);
+    }
+    // forward packs: consecutive segments of one format, <= 64 groups per pack
+    for (int f = 0; f < NFMT; ++f) {
+        VPack cur{};
+        cur.fmt = f;
+        auto flush = [&]() {
+            if (cur.nseg > 0) L.vpacks.push_back(cur);
+            cur = VPack{};
+            cur.fmt = f;
+        };
+        const int W = fmt_width(f);
+        for (size_t j = 0; j < L.vitems.size(); ++j) {
+            const VItem& v = L.vitems[j];
+            if (v.fmt != f) continue;
+            if (cur.nseg == kPackSegs || cur.gofs[cur.nseg] + v.ng > kItemGroups) flush();
+            const ColPanel& C = L.cp[v.q];
+            const int s = cur.nseg;
+            cur.gofs[s + 1] = cur.gofs[s] + v.ng;
+            cur.m[s] = C.m;
+            cur.ng[s] = v.ng;
+            cur.x0[s] = int32_t(C.col0);
+            cur.vc0[s] = int32_t(C.vcol0[f] + int64_t(v.g0) * W);
+            cur.data_off[s] = v.data_off;
+            ++cur.nseg;
+        }
+        flush();
     }
     const int64_t nvcols = vpos;
     if (nrcols > INT32_MAX || nvcols > INT32_MAX) return HMVM_ERR_UNSUPPORTED;
@@ -416,7 +490,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     L.rdest.assign(nrcols, -1);
     L.vdest.assign(nvcols, -1);
     L.vsrc.assign(nvcols, -1);
-    const bool need_scale = (opts.precision != HMVM_PREC_FP64);
+    const bool need_scale = (opts.precision != HMVM_PREC_FP64) || (opts.force_format > 0);
     if (need_scale) {
         L.zscale.assign(nrcols, 1.0f);
         L.vscale.assign(nvcols, 1.0f);
@@ -481,44 +555,50 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                     const int64_t gc = rcol0[p][f] + c;
                     L.colsrc[gc] = ~int32_t(e0 + l);
                     if (need_scale) L.zscale[gc] = sc;
-                    L.rdest[gc] = L.qt_off[e0 + l] + s.pi;
+                    L.rdest[gc] = L.qt_off[e0 + l] + s.pi * int32_t(b.rank);
                 }
             }
         }
     }
 
-    // ---- fill column panels (V rows, row-major per format) ----
+    // ---- fill column panels (V rows; item blocks row-major) ----
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t q = 0; q < NC; ++q) {
         const ColPanel& C = L.cp[q];
         const int64_t m = C.m;
         int64_t cnt[NFMT] = {0, 0, 0, 0};
-        std::vector<double> col(m);
         for (const Seg& s : cseg[q]) {
             const hmvm_block& b = blocks[s.b];
             const int64_t r0 = C.col0 - b.col_ofs;
             const int64_t e0 = L.blk_t0[s.b];
             for (int64_t l = 0; l < b.rank; ++l) {
                 const int f = L.tfmt[e0 + l];
+                const int W = fmt_width(f);
                 const int64_t c = cnt[f]++;
-                const int64_t kp = C.kpad[f];
+                // the item holding column c of format f
+                const VItem* it = nullptr;
+                for (int64_t j = C.item0; j < C.item0 + C.nitems; ++j) {
+                    const VItem& v = L.vitems[j];
+                    if (v.fmt == f && c >= int64_t(v.g0) * W && c < int64_t(v.g0 + v.ng) * W) { it = &v; break; }
+                }
+                const int64_t ld = int64_t(it->ng) * W;
+                const int64_t o = it->data_off + (c - int64_t(it->g0) * W);
                 const double* src = arena + b.off_b + l * int64_t(b.ncols) + r0;
-                const int64_t o = C.off[f] + c;
                 float sc = 1.0f;
                 switch (f) {
-                case F64: for (int64_t i = 0; i < m; ++i) L.a64[o + i * kp] = src[i]; break;
-                case F32: for (int64_t i = 0; i < m; ++i) L.a32[o + i * kp] = float(src[i]); break;
-                case BF16: for (int64_t i = 0; i < m; ++i) L.a16[o + i * kp] = encode_bf16(float(src[i])); break;
+                case F64: for (int64_t i = 0; i < m; ++i) L.a64[o + i * ld] = src[i]; break;
+                case F32: for (int64_t i = 0; i < m; ++i) L.a32[o + i * ld] = float(src[i]); break;
+                case BF16: for (int64_t i = 0; i < m; ++i) L.a16[o + i * ld] = encode_bf16(float(src[i])); break;
                 case FP8: {
                     double mx = 0;
                     for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
                     sc = fp8_scale(mx);
-                    for (int64_t i = 0; i < m; ++i) L.a8[o + i * kp] = encode_e4m3(float(src[i] / double(sc)));
+                    for (int64_t i = 0; i < m; ++i) L.a8[o + i * ld] = encode_e4m3(float(src[i] / double(sc)));
                     break;
                 }
                 }
                 const int64_t vc = C.vcol0[f] + c;
-                L.vdest[vc] = L.pt_off[e0 + l] + s.pi;
+                L.vdest[vc] = L.pt_off[e0 + l] + s.pi * int32_t(b.rank);
                 L.vsrc[vc] = int32_t(e0 + l);
                 if (need_scale) L.vscale[vc] = sc;
             }
@@ -559,6 +639,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         } else {
             const double k = b.rank;
             fl += 2 * k * (m + n) + 2 * (m + n);
+            if (b.kind == HMVM_BLOCK_LOWRANK_SV) ds += 8.0 * k;   // S (matrix/lrsvmatrix.hh:364-370)
             for (int64_t l = 0; l < b.rank; ++l) {
                 const int f = L.tfmt[L.blk_t0[i] + l];
                 ds += fmt_bytes(f) * (m + n);
@@ -627,14 +708,18 @@ int unpack_block(const HostLayout& L, int64_t bi, const void* const arenas[4], b
     for (size_t q = 0; q < L.cp.size(); ++q) {
         const ColPanel& C = L.cp[q];
         if (C.col0 < b.col_ofs || C.col0 >= b.col_ofs + b.ncols) continue;
-        for (int f = 0; f < NFMT; ++f) {
-            for (int64_t c = 0; c < C.kpad[f]; ++c) {
-                const int32_t e = L.vsrc[C.vcol0[f] + c];
+        for (int64_t j = C.item0; j < C.item0 + C.nitems; ++j) {
+            const VItem& v = L.vitems[j];
+            const int f = v.fmt, W = fmt_width(f);
+            const int64_t ld = int64_t(v.ng) * W;
+            for (int64_t cc = 0; cc < ld; ++cc) {
+                const int64_t vc = C.vcol0[f] + int64_t(v.g0) * W + cc;
+                const int32_t e = L.vsrc[vc];
                 if (e < e0 || e >= e0 + b.rank) continue;
                 const int64_t l = e - e0;
-                const double sc = (as_f64 || L.vscale.empty()) ? 1.0 : L.vscale[C.vcol0[f] + c];
+                const double sc = (as_f64 || L.vscale.empty()) ? 1.0 : L.vscale[vc];
                 for (int64_t i = 0; i < C.m; ++i)
-                    bb[l * b.ncols + (C.col0 - b.col_ofs) + i] = load_elem(f, arenas, C.off[f] + i * C.kpad[f] + c, as_f64) * sc;
+                    bb[l * b.ncols + (C.col0 - b.col_ofs) + i] = load_elem(f, arenas, v.data_off + i * ld + cc, as_f64) * sc;
             }
         }
     }

@@ -20,7 +20,8 @@ struct HostLayout {
     std::vector<RowPanel> rp;
     std::vector<RChunk> rchunks;
     std::vector<ColPanel> cp;
-    std::vector<VItem> vitems;
+    std::vector<VItem> vitems;     // V segments, panel-major
+    std::vector<VPack> vpacks;     // forward V work units
 
     // per R column (all row panels, chunk order)
     std::vector<int32_t> colsrc;   // >= 0: x index; < 0: ~t-entry; kPadCol: padding
@@ -38,6 +39,12 @@ struct HostLayout {
     std::vector<double> tmul;      // per t-entry: S_l for U S V^T leaves, else 1
     std::vector<int32_t> pt_off, pt_cnt;   // forward partials of t-entry e
     std::vector<int32_t> qt_off, qt_cnt;   // transposed partials of t-entry e
+    std::vector<int32_t> tstride;          // distance of consecutive partials of e (= rank)
+    // two-stage summation of the partials: stage 1 sums kSegLen partials per
+    // segment (seg_e, seg_r0); stage 2 sums an entry's segment results at
+    // pos0[e] + j * tstride[e], j < nseg[e]
+    std::vector<int32_t> ps_e, ps_r0, ps_pos0, ps_nseg;   // forward (P)
+    std::vector<int32_t> qs_e, qs_r0, qs_pos0, qs_nseg;   // transposed (Q)
     std::vector<int32_t> qd_off, qd_cnt;   // transposed dense partials per column
     int64_t psize = 0, qsize = 0;
 
@@ -53,6 +60,7 @@ struct HostLayout {
     int64_t cols_by_format[NFMT] = {0, 0, 0, 0};
     double flops = 0, datasize = 0, bytes_rpass = 0, bytes_vpass = 0;
     int32_t max_vitem_rows = 0;
+    int64_t max_zbytes = 0;        // largest per-row-panel z block (bytes)
 
     int64_t arena_elems(int f) const {
         return f == F64 ? int64_t(a64.size()) : f == F32 ? int64_t(a32.size())

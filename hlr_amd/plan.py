@@ -43,7 +43,7 @@ class Plan:
 
     def __init__(self, blocks: np.ndarray, arena: np.ndarray, nrows: int, ncols: int,
                  precision: str = "fp64", eps: float = 0.0, dense_prec: str = "fp32",
-                 device: int = 0, check_finite: bool = True):
+                 device: int = 0, check_finite: bool = True, force_format: str | None = None):
         self._p = None
         blocks = np.ascontiguousarray(blocks, dtype=N.block_dtype())
         arena = np.ascontiguousarray(arena, dtype=np.float64)
@@ -52,6 +52,7 @@ class Plan:
         opts.dense_prec = N.PREC_FP64 if dense_prec == "fp64" else N.PREC_FP32
         opts.eps = float(eps)
         opts.check_finite = 1 if check_finite else 0
+        opts.force_format = 0 if force_format is None else 1 + FORMAT_NAMES.index(force_format)
         p = C.c_void_p()
         rc = N.hmvm().hmvm_plan_create(blocks.ctypes.data if len(blocks) else None, len(blocks), nrows, ncols,
                                        arena.ctypes.data if arena.size else None, arena.size, C.byref(opts),

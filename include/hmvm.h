@@ -92,13 +92,23 @@ typedef struct hmvm_block {
     int64_t off_s;    /* singular values (k), or -1 when unknown   */
 } hmvm_block;
 
+/* storage formats of factor columns (index of the plan's four arenas) */
+enum {
+    HMVM_FMT_F64  = 0,
+    HMVM_FMT_F32  = 1,
+    HMVM_FMT_BF16 = 2,
+    HMVM_FMT_FP8  = 3     /* OCP FP8 E4M3 with a power-of-two scale per panel column */
+};
+
 typedef struct hmvm_plan_opts {
     int32_t precision;    /* HMVM_PREC_*                                          */
     int32_t dense_prec;   /* HMVM_PREC_FP64 or HMVM_PREC_FP32 for dense leaves
                              under HMVM_PREC_VALR (default FP32)                  */
     double  eps;          /* compression tolerance for HMVM_PREC_VALR             */
     int32_t check_finite; /* reject NaN/Inf factors at creation (default 1)       */
-    int32_t reserved;
+    int32_t force_format; /* 0: formats from `precision`; 1 + HMVM_FMT_*: store
+                             every low-rank column in that format (experiments
+                             and format tests)                                    */
 } hmvm_plan_opts;
 
 typedef struct hmvm_plan_info {

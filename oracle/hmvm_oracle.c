@@ -284,6 +284,7 @@ double oracle_mul_vec_datasize(const hmvm_block* blocks, int64_t nb, int64_t nro
     for (int64_t i = 0; i < nb; ++i) {
         const double m = blocks[i].nrows, n = blocks[i].ncols;
         if (blocks[i].kind == HMVM_BLOCK_DENSE) s += 8.0 * m * n;
+        else if (blocks[i].kind == HMVM_BLOCK_LOWRANK_SV) s += 8.0 * blocks[i].rank * (1.0 + m + n);
         else s += 8.0 * blocks[i].rank * (m + n);
     }
     return s + 2.0 * 8.0 * (double)(nrows + ncols);

@@ -131,17 +131,20 @@ def test_fp32_vectors(cuda, log1d):
     assert rel(y, yr) <= 1e-6
 
 
-@pytest.mark.parametrize("prec,eps", [("fp32", 0.0), ("valr", 1e-4), ("valr", 1e-6), ("valr", 0.5)])
+@pytest.mark.parametrize("prec,eps,force", [("fp32", 0.0, None), ("valr", 1e-4, None), ("valr", 1e-6, None),
+                                             ("valr", 1e-6, "fp8_e4m3"), ("valr", 1e-6, "bf16")])
 @pytest.mark.parametrize("op", [N.OP_NORMAL, N.OP_TRANSPOSED])
-def test_lowprec_vs_dequantized(cuda, sphere16k, prec, eps, op):
+def test_lowprec_vs_dequantized(cuda, sphere16k, prec, eps, force, op):
     """The kernels must reproduce the oracle on the stored (dequantised)
-    factors up to FP32 accumulation; eps = 0.5 puts every low-rank column
-    into FP8-E4M3, which also checks the hardware FP8 decode."""
+    factors up to FP32 accumulation.  Forcing every low-rank column into
+    FP8-E4M3 (or BF16) makes the hardware decode carry most of the result,
+    so a decode that differed from the OCP table would fail here."""
     hm = sphere16k
-    plan = Plan.from_hmatrix(hm, precision=prec, eps=eps)
+    plan = Plan.from_hmatrix(hm, precision=prec, eps=eps, force_format=force)
     info = plan.info()
-    if prec == "valr" and eps == 0.5:
-        assert info["cols_by_format"][3] == sum(info["cols_by_format"])
+    if force is not None:
+        f = ("fp64", "fp32", "bf16", "fp8_e4m3").index(force)
+        assert info["cols_by_format"][f] == sum(info["cols_by_format"])
     deq = dequantized_arena(plan, hm)
     x = random_vector(hm.n, 9)
     y = _gpu_apply(plan, 1.0, op, x, np.zeros(hm.n), cuda)
