@@ -77,15 +77,17 @@ def main() -> int:
     stream = torch.cuda.current_stream()
 
     def time_plan(plan, steps, warmup):
+        # workspace allocated once (hmvm_apply_ws): the timed loop is kernels only
+        ws = torch.empty(plan.workspace_bytes(N.OP_NORMAL, 1) // 8 + 1, dtype=torch.float64, device=dev)
         y = torch.zeros(hm.n, dtype=torch.float64, device=dev)
         for _ in range(warmup):
-            plan.apply(2.0, N.OP_NORMAL, x, y)
+            plan.apply_ws(2.0, N.OP_NORMAL, x, y, ws)
         torch.cuda.synchronize()
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
         start.record(stream)
         for _ in range(steps):
-            plan.apply(2.0, N.OP_NORMAL, x, y)
+            plan.apply_ws(2.0, N.OP_NORMAL, x, y, ws)
         stop.record(stream)
         torch.cuda.synchronize()
         ms = start.elapsed_time(stop) / steps
@@ -185,6 +187,16 @@ def main() -> int:
     gbs_mp = mvm_bytes / (msmp * 1e-3) / 1e9
     gbs_64 = i64["datasize"] / (ms64 * 1e-3) / 1e9
 
+    # HBM traffic of the same kernel from the committed rocprofv3 PMC summary
+    # (FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated), when it exists
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_invr_sphere_n{n}.txt"
+    if tfile.exists() and args.eps == 1e-6:
+        for line in tfile.read_text().splitlines():
+            f = line.split()
+            if len(f) >= 4 and not line.startswith("#") and f[0] == "valr" and f[1].startswith("k_ufwd"):
+                traffic = float(f[-2]) + float(f[-1])
+
     out = {
         "metric": "H-MVM/s + HBM GB/s (% peak), mixed-prec vs FP64 GPU at rel.err<=eps, n=262k",
         "value": 1000.0 / msmp,
@@ -217,7 +229,8 @@ def main() -> int:
         "phases_ms_fp64": ph64.tolist(),
         "roofline": {"bound": "hbm", "kernel": "k_ufwd (row-panel pass, mixed precision)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": r_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

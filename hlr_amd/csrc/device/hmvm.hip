@@ -26,6 +26,7 @@
 
 #include "hmvm.h"
 #include "kernels.hpp"
+#include "mrhs.hpp"
 #include "plan_host.hpp"
 
 using namespace hmvm;
@@ -34,6 +35,7 @@ namespace {
 
 constexpr int kThreads = 64 * kWaves;
 constexpr int64_t kWholeZBytes = 48 * 1024;   // largest panel z block staged whole in LDS
+constexpr int kXsStride = 65;                 // LDS row stride of the x slices of a V pack
 
 template <typename T> struct IsFp8 { static constexpr bool value = false; };
 template <> struct IsFp8<uint8_t> { static constexpr bool value = true; };
@@ -49,12 +51,14 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     using A = typename E::Acc;
     constexpr int W = E::W;
     const VPack& pk = *pkp;     // read through the pointer: the arrays stay out of scratch
-    A* xs = reinterpret_cast<A*>(xsraw);            // [kPackSegs][64]
+    // x slices of the segments, row stride kXsStride (odd): the lanes of
+    // different segments read different rows i of their own slice in the
+    // same step, which then fall into different LDS banks
+    A* xs = reinterpret_cast<A*>(xsraw);            // [kPackSegs][kXsStride]
     const int nseg = pk.nseg;
-    // stage the x slices of all segments: 8 lanes per segment row block
     for (int idx = lane; idx < nseg * 64; idx += 64) {
         const int j = idx >> 6, i = idx & 63;
-        if (i < pk.m[j]) xs[idx] = A(x[pk.x0[j] + i]);
+        if (i < pk.m[j]) xs[j * kXsStride + i] = A(x[pk.x0[j] + i]);
     }
     wave_sync();
     if (lane >= pk.gofs[nseg]) return;
@@ -64,7 +68,7 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     const int m = pk.m[j];
     const int64_t ld = int64_t(pk.ng[j]) * W;
     const T* base = static_cast<const T*>(P.arena[E::F]) + pk.data_off[j] + int64_t(g) * W;
-    const A* xj = xs + j * 64;
+    const A* xj = xs + j * kXsStride;
     A acc[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) acc[w] = A(0);
@@ -89,7 +93,7 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
 
 template <typename XT>
 __global__ __launch_bounds__(kThreads) void k_vfwd(DevPlan P, const XT* __restrict__ x, double* __restrict__ Pb) {
-    __shared__ double xs[kWaves][kPackSegs * 64];
+    __shared__ double xs[kWaves][kPackSegs * kXsStride];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t pi = int64_t(blockIdx.x) * kWaves + wave;
     if (pi >= P.nvpacks) return;
@@ -605,13 +609,45 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
     return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
 }
 
+// workspace of a call: the single right-hand-side layout, or the matrix-core
+// layout for forward products with several right-hand sides
+int64_t call_ws_doubles(const hmvm_plan* p, int op, int nrhs) {
+    const bool trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
+    int64_t w = ws_doubles(p, op);
+    if (!trans && nrhs > 1) w = std::max(w, mrhs_ws_doubles(p->dp, std::min(nrhs, kMrhsMax)));
+    return w;
+}
+
 int apply_impl(const hmvm_plan* p, double alpha, int op, const void* x, int64_t ldx, void* y, int64_t ldy,
                int nrhs, int vdt, void* ws, int64_t ws_bytes, hipStream_t st) {
     const bool trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
     const int64_t nin = trans ? p->L.nrows : p->L.ncols;
     const int64_t nout = trans ? p->L.ncols : p->L.nrows;
     if (nrhs > 1 && (ldx < nin || ldy < nout)) return HMVM_ERR_LAYOUT;
-    if (ws_bytes < ws_doubles(p, op) * int64_t(sizeof(double))) return HMVM_ERR_INVALID_ARG;
+    if (ws_bytes < call_ws_doubles(p, op, nrhs) * int64_t(sizeof(double))) return HMVM_ERR_INVALID_ARG;
+    const size_t es = (vdt == HMVM_F64) ? 8 : 4;
+    if (!trans && nrhs > 1) {
+        // matrix cores, up to kMrhsMax right-hand sides per pass
+        for (int r0 = 0; r0 < nrhs; r0 += kMrhsMax) {
+            const int R = std::min(kMrhsMax, nrhs - r0);
+            const void* xr = static_cast<const char*>(x) + size_t(r0) * size_t(ldx) * es;
+            void* yr = static_cast<char*>(y) + size_t(r0) * size_t(ldy) * es;
+            int rc;
+            if (R == 1) {
+                rc = (vdt == HMVM_F64)
+                         ? launch_apply(p, alpha, op, static_cast<const double*>(xr), static_cast<double*>(yr),
+                                        static_cast<double*>(ws), st)
+                         : launch_apply(p, alpha, op, static_cast<const float*>(xr), static_cast<float*>(yr),
+                                        static_cast<double*>(ws), st);
+            } else {
+                rc = launch_apply_mrhs(p->dp, 0, alpha, xr, ldx, yr, ldy, R, vdt, static_cast<double*>(ws), st) == 0
+                         ? HMVM_OK : HMVM_ERR_DEVICE;
+            }
+            if (rc != HMVM_OK) return rc;
+        }
+        return HMVM_OK;
+    }
+    // transposed (and single right-hand side): one vector at a time
     for (int r = 0; r < nrhs; ++r) {
         int rc;
         if (vdt == HMVM_F64)
@@ -737,8 +773,7 @@ int hmvm_plan_get_info(const hmvm_plan* p, hmvm_plan_info* info) {
 
 int hmvm_workspace_size(const hmvm_plan* p, int op, int nrhs, int64_t* bytes) {
     if (!p || !bytes || nrhs < 1) return HMVM_ERR_INVALID_ARG;
-    (void)nrhs;
-    *bytes = ws_doubles(p, op) * int64_t(sizeof(double));
+    *bytes = call_ws_doubles(p, op, nrhs) * int64_t(sizeof(double));
     return HMVM_OK;
 }
 
@@ -761,7 +796,7 @@ int hmvm_apply(const hmvm_plan* p, double alpha, int op, const void* x, int64_t 
     DeviceGuard g(p->device);
     if (!g.ok) return HMVM_ERR_DEVICE;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int64_t bytes = ws_doubles(p, op) * int64_t(sizeof(double));
+    const int64_t bytes = call_ws_doubles(p, op, nrhs) * int64_t(sizeof(double));
     void* ws = nullptr;
     if (hipMallocAsync(&ws, size_t(bytes), st) != hipSuccess) return HMVM_ERR_ALLOC;
     rc = apply_impl(p, alpha, op, x, ldx, y, ldy, nrhs, vdt, ws, bytes, st);

@@ -432,21 +432,19 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             C.vcol0[f] = vpos;
             vpos += kp;
             const int64_t ng = kp / W;
-            // items of <= kItemGroups groups, as even as possible, each one
-            // contiguous m x (ng * W) row-major block
-            if (ng > 0) {
-                const int64_t nit = (ng + kItemGroups - 1) / kItemGroups;
-                const int64_t base = ng / nit, extra = ng % nit;
-                int64_t g = 0;
-                for (int64_t it = 0; it < nit; ++it) {
-                    const int64_t cnt = base + (it < extra ? 1 : 0);
-                    VItem v;
-                    v.q = int32_t(q); v.fmt = int32_t(f); v.g0 = int32_t(g); v.ng = int32_t(cnt);
-                    v.data_off = varena_pos[f];
-                    varena_pos[f] += int64_t(C.m) * cnt * W;
-                    L.vitems.push_back(v);
-                    g += cnt;
-                }
+            // segments of kItemGroups groups (one lane each) plus a remainder
+            // that the forward packs merge with remainders of other panels;
+            // each segment is a contiguous m x (ng * W) row-major block that
+            // starts on a 128-byte boundary
+            const int64_t align = 128 / fmt_bytes(f);
+            for (int64_t g = 0; g < ng; g += kItemGroups) {
+                const int64_t cnt = std::min<int64_t>(kItemGroups, ng - g);
+                varena_pos[f] = round_up(varena_pos[f], align);
+                VItem v;
+                v.q = int32_t(q); v.fmt = int32_t(f); v.g0 = int32_t(g); v.ng = int32_t(cnt);
+                v.data_off = varena_pos[f];
+                varena_pos[f] += int64_t(C.m) * cnt * W;
+                L.vitems.push_back(v);
             }
         }
         C.nitems = int32_t(int64_t(L.vitems.size()) - C.item0);

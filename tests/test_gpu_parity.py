@@ -121,6 +121,30 @@ def test_multi_rhs(cuda, log1d):
         assert rel(Yt.cpu().numpy(), Yr) <= TOL_FP64
 
 
+@pytest.mark.parametrize("nrhs", [2, 16, 33, 64, 70])
+@pytest.mark.parametrize("prec", ["fp64", "valr"])
+def test_multi_rhs_matrix_cores(cuda, sphere16k, prec, nrhs):
+    """Forward products with several right-hand sides run on the matrix
+    cores (16 x 16 x 4 MFMA tiles, rhs padded to 16, > 64 split)."""
+    import torch
+    hm = sphere16k
+    plan = Plan.from_hmatrix(hm, precision=prec, eps=1e-6)
+    arena = hm.arena if prec == "fp64" else dequantized_arena(plan, hm)
+    X = np.stack([random_vector(hm.n, 100 + r) for r in range(nrhs)])
+    Y0 = np.stack([random_vector(hm.n, 200 + r) for r in range(nrhs)])
+    Xt = torch.as_tensor(X, device=cuda)
+    Yt = torch.as_tensor(Y0, device=cuda).clone()
+    print(f"[gpu] mrhs prec={prec} nrhs={nrhs}", flush=True)
+    plan.apply(0.5, N.OP_NORMAL, Xt, Yt)
+    torch.cuda.synchronize()
+    Yr = Y0.copy()
+    oracle.mul_vec(hm.blocks, arena, 0.5, N.OP_NORMAL, X, Yr)
+    got = Yt.cpu().numpy()
+    tol = TOL_FP64 if prec == "fp64" else TOL_FP32ACC
+    for r in range(nrhs):
+        assert rel(got[r] - Y0[r], Yr[r] - Y0[r]) <= tol, r
+
+
 def test_fp32_vectors(cuda, log1d):
     import torch
     hm = log1d

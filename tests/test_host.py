@@ -1,0 +1,178 @@
+"""Host-side checks that need no GPU: the C-ABI libraries load and export
+every declared symbol, plan layout and storage precision on host-only plans
+(device = -1) match the oracle bit for bit, and invalid input is rejected
+with the documented error codes."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from hlr_amd import _native as N
+from hlr_amd.hmatrix import HMatrix, INVR_SPHERE, LOG1D, MATERN_TLR
+from hlr_amd.plan import Plan
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared(header: str):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:hmvm|hlrgen)_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("hmvm.h", "libhmvm.so"), ("hlrgen.h", "libhlrgen.so")])
+def test_exports(header, lib):
+    so = C.CDLL(str(ROOT / "hlr_amd" / "lib" / lib))
+    names = _declared(header)
+    assert len(names) > 5
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+
+
+def test_python_bindings_cover_header():
+    names = _declared("hmvm.h")
+    bound = {s[0] for s in N.HMVM_SYMBOLS}
+    assert set(names) <= bound, set(names) - bound
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    return HMatrix(INVR_SPHERE.with_(n=4096))
+
+
+def test_fp64_layout_roundtrip(sphere):
+    p = Plan.from_hmatrix(sphere, precision="fp64", device=-1)
+    ar = [p.arena_numpy(f) for f in range(4)]
+    for b, blk in enumerate(sphere.blocks):
+        m, n, k = int(blk["nrows"]), int(blk["ncols"]), int(blk["rank"])
+        A, B = p.unpack_block(b, ar)
+        if int(blk["kind"]) == N.BLOCK_DENSE:
+            assert np.array_equal(A, sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T)
+        elif k:
+            assert np.array_equal(A, sphere.arena[blk["off_a"]:blk["off_a"] + m * k].reshape(k, m).T)
+            assert np.array_equal(B, sphere.arena[blk["off_b"]:blk["off_b"] + n * k].reshape(k, n).T)
+    info = p.info()
+    assert info["datasize"] == pytest.approx(oracle.mul_vec_datasize(sphere.blocks, sphere.n, sphere.n))
+    assert info["flops"] == pytest.approx(oracle.mul_vec_flops(sphere.blocks))
+
+
+@pytest.mark.parametrize("eps", [1e-4, 1e-6, 1e-8])
+def test_valr_chooser_and_quantisation(sphere, eps):
+    """Per-column formats follow the oracle's VALR chooser; every stored
+    value equals the oracle's rounding of the FP64 value (after the
+    power-of-two scale for FP8), bit for bit."""
+    p = Plan.from_hmatrix(sphere, precision="valr", eps=eps, device=-1)
+    ar = [p.arena_numpy(f) for f in range(4)]
+    counts = [0, 0, 0, 0]
+    rng = np.random.default_rng(3)
+    for b, blk in enumerate(sphere.blocks):
+        m, n, k = int(blk["nrows"]), int(blk["ncols"]), int(blk["rank"])
+        if int(blk["kind"]) == N.BLOCK_DENSE:
+            continue
+        S = sphere.arena[blk["off_s"]:blk["off_s"] + k]
+        fm = [oracle.format_for_rate(oracle.eps_to_rate(u)) for u in oracle.valr_tolerances(S, eps)]
+        for f in fm:
+            counts[f] += 1
+        if rng.random() > 0.1:
+            continue
+        U0 = sphere.arena[blk["off_a"]:blk["off_a"] + m * k].reshape(k, m).T
+        V0 = sphere.arena[blk["off_b"]:blk["off_b"] + n * k].reshape(k, n).T
+        U, V = p.unpack_block(b, ar)
+        for l, f in enumerate(fm):
+            for got, ref, rows in ((U[:, l], U0[:, l], m), (V[:, l], V0[:, l], n)):
+                if f == 1:
+                    exp = ref.astype(np.float32).astype(np.float64)
+                elif f == 2:
+                    exp = np.array([oracle.round_bf16(v) for v in ref])
+                elif f == 3:
+                    # scale per 64-row panel column: largest power of two step with max|v|/s <= 448
+                    exp = np.empty(rows)
+                    for r0 in range(0, rows, 64):
+                        seg = ref[r0:r0 + 64]
+                        mx = np.abs(seg).max()
+                        s = 2.0 ** np.ceil(np.log2(mx / 448.0)) if mx > 0 else 1.0
+                        if mx / s > 448:
+                            s *= 2
+                        exp[r0:r0 + 64] = [oracle.round_e4m3(float(np.float32(v / s))) * s for v in seg]
+                else:
+                    exp = ref
+                assert np.array_equal(got, exp), (b, l, f)
+    assert counts == p.info()["cols_by_format"]
+
+
+def test_dense_and_datasize_valr(sphere):
+    p = Plan.from_hmatrix(sphere, precision="valr", eps=1e-6, device=-1)
+    ar = [p.arena_numpy(f) for f in range(4)]
+    for b, blk in enumerate(sphere.blocks[:200]):
+        if int(blk["kind"]) == N.BLOCK_DENSE:
+            m, n = int(blk["nrows"]), int(blk["ncols"])
+            D, _ = p.unpack_block(b, ar)
+            ref = sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T
+            assert np.array_equal(D, ref.astype(np.float32).astype(np.float64))
+
+
+def test_tlr_one_level():
+    hm = HMatrix(MATERN_TLR.with_(n=2048, ntile=256))
+    assert Plan.from_hmatrix(hm, device=-1).info()["one_level"] == 1
+    hm2 = HMatrix(LOG1D.with_(n=1024))
+    assert Plan.from_hmatrix(hm2, device=-1).info()["one_level"] == 0
+
+
+# --- error handling ----------------------------------------------------------
+
+def _one_block(**kw):
+    bt = N.block_dtype()
+    b = np.zeros(1, dtype=bt)
+    b["row_ofs"], b["col_ofs"], b["nrows"], b["ncols"] = 0, 0, 4, 4
+    b["kind"], b["rank"], b["off_a"], b["off_b"], b["off_s"] = N.BLOCK_LOWRANK, 2, 0, 8, -1
+    for k, v in kw.items():
+        b[k] = v
+    return b
+
+
+@pytest.mark.parametrize("kw,code", [
+    ({"rank": 5}, 2),                      # rank > min(m, n)
+    ({"row_ofs": 2}, 2),                   # block leaves the matrix
+    ({"off_b": 100}, 2),                   # factor outside the arena
+    ({"kind": 7}, 2),                      # unknown leaf kind
+    ({"nrows": -1}, 2),
+])
+def test_bad_blocks(kw, code):
+    arena = np.ones(16)
+    with pytest.raises(N.HmvmError) as e:
+        Plan(_one_block(**kw), arena, 4, 4, device=-1)
+    assert e.value.code == code
+
+
+def test_nonfinite_and_overlap():
+    arena = np.ones(16)
+    arena[3] = np.nan
+    with pytest.raises(N.HmvmError) as e:
+        Plan(_one_block(), arena, 4, 4, device=-1)
+    assert e.value.code == 4
+    Plan(_one_block(), arena, 4, 4, device=-1, check_finite=False)      # allowed when unchecked
+    two = np.concatenate([_one_block(), _one_block()])
+    with pytest.raises(N.HmvmError) as e:
+        Plan(two, np.ones(16), 4, 4, device=-1)
+    assert e.value.code == 3
+
+
+def test_empty_and_rank0():
+    # no leaves at all, and a rank-0 leaf: valid plans with nothing to do
+    Plan(np.zeros(0, dtype=N.block_dtype()), np.zeros(0), 8, 8, device=-1)
+    p = Plan(_one_block(rank=0), np.zeros(16), 4, 4, device=-1)
+    assert p.info()["nlowrank"] == 1
+
+
+def test_apply_on_host_plan_rejected():
+    p = Plan(_one_block(), np.ones(16), 4, 4, device=-1)
+    rc = N.hmvm().hmvm_apply(p.handle, 1.0, 0, C.c_void_p(8), 0, C.c_void_p(8), 0, 1, N.F64, None)
+    assert rc == 1
+    rc = N.hmvm().hmvm_apply(None, 1.0, 0, None, 0, None, 0, 1, N.F64, None)
+    assert rc == 1
+    assert N.error_string(3) == "leaf blocks overlap"
