@@ -106,6 +106,7 @@ HMVM_SYMBOLS = [
     ("hmvm_batch_capture", _i32, [_vp, C.POINTER(C.c_double), C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_vp),
                                   C.POINTER(_i64), _vp]),
     ("hmvm_batch_replay", _i32, [_vp, _vp]),
+    ("hmvm_batch_check", _i32, [_vp, _vp]),
     ("hmvm_batch_tune", _i32, [_vp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_vp), C.POINTER(_i64), _vp,
                                C.POINTER(_i32)]),
     ("hmvm_error_string", C.c_char_p, [_i32]),

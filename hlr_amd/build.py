@@ -49,7 +49,7 @@ def build_hmvm() -> Path:
     LIBDIR.mkdir(exist_ok=True)
     dev = CSRC / "device"
     objs = []
-    for src in ("plan_build.cpp", "hmvm.hip", "mrhs.hip"):
+    for src in ("plan_build.cpp", "hmvm.hip", "mrhs.hip", "batch.hip"):
         obj = LIBDIR / (Path(src).stem + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         flags = DEVICE_FLAGS + [f"--offload-arch={ARCH}"] if src.endswith(".hip") else []
@@ -65,8 +65,18 @@ def build_hmvm() -> Path:
     return out
 
 
+def build_driver() -> Path:
+    """mvm-hip: benchmark driver over the C++ mirror hlr::hip::mul_vec"""
+    out = LIBDIR / "mvm-hip"
+    tmp = LIBDIR / "mvm-hip.tmp"
+    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC / 'host'}", str(CSRC / "host" / "mvm_hip.cpp"),
+          f"-L{LIBDIR}", "-lhmvm", "-lhlrgen", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> list[Path]:
-    return [build_hlrgen(), build_hmvm()]
+    return [build_hlrgen(), build_hmvm(), build_driver()]
 
 
 if __name__ == "__main__":

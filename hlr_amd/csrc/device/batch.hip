@@ -1,0 +1,467 @@
+// batch.hip -- forward products of many H-matrices and right-hand sides in
+// one fixed sequence of launches.
+//
+// Every member keeps its own plan.  A batch lists the work of all members as
+// units of six kinds; one kernel per kind runs every unit of that kind of
+// every member, one unit per workgroup:
+//   V1  column-panel pass of members with one right-hand side (4 packs, one
+//       per wave)
+//   VM  column-panel pass on the matrix cores (one segment, nrhs >= 2)
+//   P1  first stage of the partial sums (a range of (segment, rhs) items)
+//   P2  second stage (a range of (entry, rhs) items)
+//   R1  row-panel pass, one right-hand side (one row panel)
+//   RM  row-panel pass on the matrix cores (one row panel, nrhs >= 2)
+// The launches are ordered on the caller's stream, which is the only
+// synchronisation: no workgroup waits for another.  The launch count is six
+// whatever the number of members, and the whole sequence (descriptor upload
+// plus the six kernels) can be captured in a HIP graph and replayed.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "fwd.hpp"
+#include "hmvm.h"
+#include "internal.hpp"
+#include "mrhs_dev.hpp"
+
+using namespace hmvm;
+
+namespace {
+
+enum UnitKind : int { K_V1 = 0, K_VM, K_P1, K_P2, K_R1, K_RM, kKinds };
+constexpr int kConfigs = 4;                     // {member order, largest first} x {1024, 4096} sum items
+constexpr int kSumItems[2] = {1024, 4096};      // (segment | entry, rhs) items per P1 / P2 unit
+
+struct BatchMat {
+    DevPlan D;
+    const void* x;
+    void* y;
+    int64_t ldx, ldy;
+    double alpha;
+    double* part;
+    double* tmp;
+    double* t;
+    int32_t R;
+    int32_t pad_;
+};
+
+struct BatchUnit {
+    int32_t mat, first, count, pad_;
+};
+
+template <typename XT>
+__global__ __launch_bounds__(kThreads) void k_bv1(const BatchMat* __restrict__ mats,
+                                                  const BatchUnit* __restrict__ units) {
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BatchUnit w = units[blockIdx.x];
+    const BatchMat& M = mats[w.mat];
+    if (wave >= w.count) return;
+    const VPack* pk = M.D.vpacks + w.first + wave;
+    const XT* x = static_cast<const XT*>(M.x);
+    double* xs = lds + wave * kPackSegs * kXsStride;
+    switch (pk->fmt) {
+    case F64: vpack_fwd<double>(M.D, pk, x, M.part, xs, lane); break;
+    case F32: vpack_fwd<float>(M.D, pk, x, M.part, xs, lane); break;
+    case BF16: vpack_fwd<uint16_t>(M.D, pk, x, M.part, xs, lane); break;
+    default: vpack_fwd<uint8_t>(M.D, pk, x, M.part, xs, lane); break;
+    }
+}
+
+template <typename XT>
+__global__ __launch_bounds__(kThreads) void k_bvm(const BatchMat* __restrict__ mats,
+                                                  const BatchUnit* __restrict__ units) {
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BatchUnit w = units[blockIdx.x];
+    const BatchMat& M = mats[w.mat];
+    vseg_mm_dispatch(M.D, M.D.vitems[w.first], static_cast<const XT*>(M.x), M.ldx, M.R, M.part, lds, tid, wave,
+                     lane);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bp1(const BatchMat* __restrict__ mats,
+                                                  const BatchUnit* __restrict__ units) {
+    const BatchUnit w = units[blockIdx.x];
+    const BatchMat& M = mats[w.mat];
+    const int64_t g1 = int64_t(w.first) + w.count;
+    for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads) psum1_item(M.D, M.part, M.tmp, M.R, g);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bp2(const BatchMat* __restrict__ mats,
+                                                  const BatchUnit* __restrict__ units) {
+    const BatchUnit w = units[blockIdx.x];
+    const BatchMat& M = mats[w.mat];
+    const int64_t g1 = int64_t(w.first) + w.count;
+    for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads) psum2_item(M.D, M.tmp, M.t, M.R, M.alpha, g);
+}
+
+template <typename XT, typename YT>
+__global__ __launch_bounds__(kThreads) void k_br1(const BatchMat* __restrict__ mats,
+                                                  const BatchUnit* __restrict__ units) {
+    extern __shared__ double lds[];
+    __shared__ double red[kWaves][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BatchUnit w = units[blockIdx.x];
+    const BatchMat& M = mats[w.mat];
+    const RowPanel Rw = M.D.rp[w.first];
+    const double acc = rpanel_fwd_chunked(M.D, Rw, M.alpha, static_cast<const XT*>(M.x), M.t, lds, tid, wave, lane);
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && lane < Rw.m) {
+        YT* y = static_cast<YT*>(M.y);
+        const double s = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+        y[Rw.row0 + lane] = YT(double(y[Rw.row0 + lane]) + s);
+    }
+}
+
+template <typename XT, typename YT>
+__global__ __launch_bounds__(kThreads) void k_brm(const BatchMat* __restrict__ mats,
+                                                  const BatchUnit* __restrict__ units) {
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const BatchUnit w = units[blockIdx.x];
+    const BatchMat& M = mats[w.mat];
+    const RowPanel Rw = M.D.rp[w.first];
+    rpanel_mm(M.D, Rw, M.alpha, static_cast<const XT*>(M.x), M.ldx, M.t, static_cast<YT*>(M.y), M.ldy, M.R, lds,
+              tid, wave, lane);
+}
+
+inline int64_t pad32(int64_t v) { return ((v + 31) / 32) * 32; }
+
+} // namespace
+
+struct hmvm_batch {
+    int device = -1;
+    int vdt = HMVM_F64;
+    int nmat = 0;
+    std::vector<const hmvm_plan*> plans;
+    std::vector<int> nrhs;
+    std::vector<BatchMat> proto;            // per member, pointers to vectors filled at apply time
+    BatchMat* h_mats = nullptr;             // pinned host staging of the descriptors
+    BatchMat* d_mats = nullptr;
+    // per configuration: the units of all kinds back to back, kind k at
+    // kind_ofs[c][k] .. kind_ofs[c][k + 1]
+    std::vector<BatchUnit> units[kConfigs];
+    int64_t kind_ofs[kConfigs][kKinds + 1] = {};
+    BatchUnit* d_units[kConfigs] = {};
+    size_t lds[kKinds] = {};
+    int config = 1;
+    double* d_ws = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+struct Guard {
+    int prev = -1;
+    bool ok = true;
+    explicit Guard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) { ok = false; return; }
+        if (prev != dev && hipSetDevice(dev) != hipSuccess) ok = false;
+    }
+    ~Guard() { if (prev >= 0) (void)hipSetDevice(prev); }
+};
+
+int fill_descriptors(hmvm_batch* b, const double* alphas, const void* const* xs, const int64_t* ldx,
+                     void* const* ys, const int64_t* ldy) {
+    for (int i = 0; i < b->nmat; ++i) {
+        if (!xs[i] || !ys[i]) return HMVM_ERR_INVALID_ARG;
+        const HostLayout& L = plan_layout(b->plans[i]);
+        const int R = b->nrhs[i];
+        const int64_t lx = ldx ? ldx[i] : L.ncols, ly = ldy ? ldy[i] : L.nrows;
+        if (R > 1 && (lx < L.ncols || ly < L.nrows)) return HMVM_ERR_LAYOUT;
+        BatchMat m = b->proto[i];
+        m.x = xs[i];
+        m.y = ys[i];
+        m.ldx = lx;
+        m.ldy = ly;
+        m.alpha = alphas ? alphas[i] : 1.0;
+        b->h_mats[i] = m;
+    }
+    return HMVM_OK;
+}
+
+template <typename XT, typename YT>
+void launch_kinds(hmvm_batch* b, hipStream_t st) {
+    const int c = b->config;
+    const BatchUnit* u = b->d_units[c];
+    const int64_t* o = b->kind_ofs[c];
+    auto grid = [&](int k) { return unsigned(o[k + 1] - o[k]); };
+    if (grid(K_V1)) k_bv1<XT><<<grid(K_V1), kThreads, b->lds[K_V1], st>>>(b->d_mats, u + o[K_V1]);
+    if (grid(K_VM)) k_bvm<XT><<<grid(K_VM), kThreads, b->lds[K_VM], st>>>(b->d_mats, u + o[K_VM]);
+    if (grid(K_P1)) k_bp1<<<grid(K_P1), kThreads, 0, st>>>(b->d_mats, u + o[K_P1]);
+    if (grid(K_P2)) k_bp2<<<grid(K_P2), kThreads, 0, st>>>(b->d_mats, u + o[K_P2]);
+    if (grid(K_R1)) k_br1<XT, YT><<<grid(K_R1), kThreads, b->lds[K_R1], st>>>(b->d_mats, u + o[K_R1]);
+    if (grid(K_RM)) k_brm<XT, YT><<<grid(K_RM), kThreads, b->lds[K_RM], st>>>(b->d_mats, u + o[K_RM]);
+}
+
+int launch(hmvm_batch* b, hipStream_t st) {
+    if (hipMemcpyAsync(b->d_mats, b->h_mats, sizeof(BatchMat) * size_t(b->nmat), hipMemcpyHostToDevice, st) !=
+        hipSuccess)
+        return HMVM_ERR_DEVICE;
+    if (b->vdt == HMVM_F64)
+        launch_kinds<double, double>(b, st);
+    else
+        launch_kinds<float, float>(b, st);
+    return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
+}
+
+template <typename V>
+int dev_alloc(hmvm_batch* b, V** out, size_t n) {
+    *out = nullptr;
+    if (n == 0) return HMVM_OK;
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(V)) != hipSuccess) return HMVM_ERR_ALLOC;
+    b->allocs.push_back(p);
+    *out = static_cast<V*>(p);
+    return HMVM_OK;
+}
+
+void release(hmvm_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->h_mats) (void)hipHostFree(b->h_mats);
+}
+
+} // namespace
+
+extern "C" {
+
+int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs, int vdt, hmvm_batch** out) {
+    if (!out) return HMVM_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!plans || nplans < 1 || (vdt != HMVM_F64 && vdt != HMVM_F32)) return HMVM_ERR_INVALID_ARG;
+    const int dev = plan_device(plans[0]);
+    for (int i = 0; i < nplans; ++i) {
+        if (!plans[i] || plan_device(plans[i]) != dev || dev < 0) return HMVM_ERR_INVALID_ARG;
+        const int R = nrhs ? nrhs[i] : 1;
+        if (R < 1) return HMVM_ERR_INVALID_ARG;
+        if (R > kMaxRp) return HMVM_ERR_UNSUPPORTED;
+    }
+    std::unique_ptr<hmvm_batch> b;
+    try {
+        b.reset(new hmvm_batch);
+    } catch (...) {
+        return HMVM_ERR_ALLOC;
+    }
+    b->device = dev;
+    b->vdt = vdt;
+    b->nmat = nplans;
+    b->plans.assign(plans, plans + nplans);
+    for (int i = 0; i < nplans; ++i) b->nrhs.push_back(nrhs ? nrhs[i] : 1);
+    Guard g(dev);
+    if (!g.ok) return HMVM_ERR_DEVICE;
+
+    // per-member descriptors and workspaces; units of every kind with their
+    // algorithmic bytes (the sort key of the largest-first order)
+    struct Sized { BatchUnit u; double bytes; };
+    std::vector<Sized> kinds[2][kKinds];        // [sum granularity][kind]
+    std::vector<int64_t> wsoff(nplans);
+    int64_t wsz = 0;
+    int maxRp = 16;
+    for (int i = 0; i < nplans; ++i) {
+        const DevPlan& D = plan_devplan(plans[i]);
+        const HostLayout& L = plan_layout(plans[i]);
+        const int R = b->nrhs[i];
+        if (R > 1) maxRp = std::max(maxRp, (R + 15) & ~15);
+        BatchMat m{};
+        m.D = D;
+        m.R = R;
+        wsoff[i] = wsz;
+        wsz += pad32(D.psize * R) + pad32(D.nps * R) + pad32(D.ntent * R) + 32;
+        for (int gsel = 0; gsel < 2; ++gsel) {
+            auto& K = kinds[gsel];
+            if (R == 1) {
+                for (int64_t k = 0; k < D.nvpacks; k += kWaves) {
+                    const int cnt = int(std::min<int64_t>(kWaves, D.nvpacks - k));
+                    double by = 0;
+                    for (int j = 0; j < cnt; ++j) {
+                        const VPack& pk = L.vpacks[k + j];
+                        by += double(pk.gofs[pk.nseg]) * 64.0 * 16.0;
+                    }
+                    K[K_V1].push_back({{i, int32_t(k), cnt, 0}, by});
+                }
+            } else {
+                for (int64_t k = 0; k < D.nvitems; ++k)
+                    K[K_VM].push_back({{i, int32_t(k), 1, 0}, double(L.vitems[k].ng) * 16.0 * 64.0 * R});
+            }
+            const int items = kSumItems[gsel];
+            for (int64_t k = 0; k < D.nps * R; k += items)
+                K[K_P1].push_back({{i, int32_t(k), int32_t(std::min<int64_t>(items, D.nps * R - k)), 0},
+                                   double(std::min<int64_t>(items, D.nps * R - k))});
+            for (int64_t k = 0; k < D.ntent * R; k += items)
+                K[K_P2].push_back({{i, int32_t(k), int32_t(std::min<int64_t>(items, D.ntent * R - k)), 0},
+                                   double(std::min<int64_t>(items, D.ntent * R - k))});
+            for (int64_t p = 0; p < D.nrp; ++p) {
+                const RowPanel& rp = L.rp[p];
+                double by = 0;
+                for (int64_t c = rp.chunk0; c < rp.chunk0 + rp.nchunks; ++c)
+                    by += double(L.rchunks[c].ngroups) * rp.m * 16.0;
+                K[R == 1 ? K_R1 : K_RM].push_back({{i, int32_t(p), 1, 0}, by * R});
+            }
+        }
+        b->proto.push_back(m);
+    }
+    // configuration c: order = c / 2 (0 member order, 1 largest first),
+    // sum granularity = c % 2
+    for (int c = 0; c < kConfigs; ++c) {
+        const int order = c / 2, gsel = c % 2;
+        b->kind_ofs[c][0] = 0;
+        for (int k = 0; k < kKinds; ++k) {
+            std::vector<Sized> v = kinds[gsel][k];
+            if (order == 1)
+                std::stable_sort(v.begin(), v.end(), [](const Sized& a, const Sized& z) { return a.bytes > z.bytes; });
+            for (const Sized& s : v) b->units[c].push_back(s.u);
+            b->kind_ofs[c][k + 1] = int64_t(b->units[c].size());
+        }
+    }
+    b->lds[K_V1] = size_t(kWaves) * kPackSegs * kXsStride * sizeof(double);
+    b->lds[K_VM] = size_t(64) * maxRp * sizeof(double);
+    b->lds[K_R1] = size_t(kChunkGroups) * 16 * sizeof(float);
+    b->lds[K_RM] = size_t(kKB) * maxRp * sizeof(double);
+
+    int rc;
+    if ((rc = dev_alloc(b.get(), &b->d_mats, size_t(nplans))) != HMVM_OK) { release(b.get()); return rc; }
+    for (int c = 0; c < kConfigs; ++c)
+        if ((rc = dev_alloc(b.get(), &b->d_units[c], b->units[c].size())) != HMVM_OK) { release(b.get()); return rc; }
+    if ((rc = dev_alloc(b.get(), &b->d_ws, size_t(wsz))) != HMVM_OK) { release(b.get()); return rc; }
+    if (hipHostMalloc(reinterpret_cast<void**>(&b->h_mats), sizeof(BatchMat) * size_t(nplans)) != hipSuccess) {
+        b->h_mats = nullptr;
+        release(b.get());
+        return HMVM_ERR_ALLOC;
+    }
+    for (int c = 0; c < kConfigs; ++c)
+        if (!b->units[c].empty() &&
+            hipMemcpy(b->d_units[c], b->units[c].data(), sizeof(BatchUnit) * b->units[c].size(),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            release(b.get());
+            return HMVM_ERR_DEVICE;
+        }
+    for (int i = 0; i < nplans; ++i) {
+        const DevPlan& D = b->proto[i].D;
+        const int R = b->proto[i].R;
+        double* base = b->d_ws + wsoff[i];
+        b->proto[i].part = base;
+        b->proto[i].tmp = base + pad32(D.psize * R);
+        b->proto[i].t = b->proto[i].tmp + pad32(D.nps * R);
+    }
+    *out = b.release();
+    return HMVM_OK;
+}
+
+int hmvm_batch_destroy(hmvm_batch* b) {
+    if (!b) return HMVM_OK;
+    {
+        Guard g(b->device);
+        (void)hipDeviceSynchronize();
+        release(b);
+    }
+    delete b;
+    return HMVM_OK;
+}
+
+int hmvm_batch_apply(hmvm_batch* b, const double* alphas, const void* const* xs, const int64_t* ldx,
+                     void* const* ys, const int64_t* ldy, void* stream) {
+    if (!b || !xs || !ys) return HMVM_ERR_INVALID_ARG;
+    Guard g(b->device);
+    if (!g.ok) return HMVM_ERR_DEVICE;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the pinned descriptors may still be read by an earlier launch
+    if (hipStreamSynchronize(st) != hipSuccess) return HMVM_ERR_DEVICE;
+    const int rc = fill_descriptors(b, alphas, xs, ldx, ys, ldy);
+    if (rc != HMVM_OK) return rc;
+    return launch(b, st);
+}
+
+int hmvm_batch_capture(hmvm_batch* b, const double* alphas, const void* const* xs, const int64_t* ldx,
+                       void* const* ys, const int64_t* ldy, void* stream) {
+    if (!b || !xs || !ys) return HMVM_ERR_INVALID_ARG;
+    Guard g(b->device);
+    if (!g.ok) return HMVM_ERR_DEVICE;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (hipStreamSynchronize(st) != hipSuccess) return HMVM_ERR_DEVICE;
+    int rc = fill_descriptors(b, alphas, xs, ldx, ys, ldy);
+    if (rc != HMVM_OK) return rc;
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return HMVM_ERR_DEVICE;
+    rc = launch(b, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc != HMVM_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != HMVM_OK ? rc : HMVM_ERR_DEVICE;
+    }
+    b->graph = graph;
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+        b->exec = nullptr;
+        return HMVM_ERR_DEVICE;
+    }
+    return HMVM_OK;
+}
+
+int hmvm_batch_replay(hmvm_batch* b, void* stream) {
+    if (!b || !b->exec) return HMVM_ERR_INVALID_ARG;
+    Guard g(b->device);
+    if (!g.ok) return HMVM_ERR_DEVICE;
+    return hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
+}
+
+int hmvm_batch_check(hmvm_batch* b, void* stream) {
+    if (!b) return HMVM_ERR_INVALID_ARG;
+    Guard g(b->device);
+    if (!g.ok) return HMVM_ERR_DEVICE;
+    return hipStreamSynchronize(static_cast<hipStream_t>(stream)) == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
+}
+
+int hmvm_batch_tune(hmvm_batch* b, const void* const* xs, const int64_t* ldx, void* const* ys, const int64_t* ldy,
+                    void* stream, int* chosen) {
+    if (!b || !xs || !ys) return HMVM_ERR_INVALID_ARG;
+    Guard g(b->device);
+    if (!g.ok) return HMVM_ERR_DEVICE;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (hipStreamSynchronize(st) != hipSuccess) return HMVM_ERR_DEVICE;
+    // every candidate accumulates alpha = 1 products into the caller's y,
+    // which the caller resets afterwards
+    int rc = fill_descriptors(b, nullptr, xs, ldx, ys, ldy);
+    if (rc != HMVM_OK) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return HMVM_ERR_DEVICE;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return HMVM_ERR_DEVICE; }
+    const int keep = b->config;
+    float best = 1e30f;
+    int best_cfg = -1;
+    for (int c = 0; c < kConfigs && rc == HMVM_OK; ++c) {
+        b->config = c;
+        if ((rc = launch(b, st)) != HMVM_OK) break;          // warm-up
+        float tot = 0;
+        for (int rep = 0; rep < 3 && rc == HMVM_OK; ++rep) {
+            if (hipEventRecord(e0, st) != hipSuccess) { rc = HMVM_ERR_DEVICE; break; }
+            rc = launch(b, st);
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                rc = HMVM_ERR_DEVICE;
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tot += ms;
+        }
+        if (rc == HMVM_OK && tot < best) { best = tot; best_cfg = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != HMVM_OK) { b->config = keep; return rc; }
+    b->config = best_cfg;
+    if (chosen) *chosen = best_cfg;
+    return HMVM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,157 @@
+// hlr_hip.hh -- C++ mirror of HLR's mul_vec call surface over the hmvm C-ABI.
+//
+// Header-only.  Gives C++ callers the reference's spelling:
+//   hlr::hip::mul_vec(alpha, op, M, x, y)      (tbb/arith.hh:312-340,
+//                                               seq/arith.hh:40-72)
+//   hlr::hip::tlr::mul_vec(alpha, op, M, x, y) (tbb/arith.hh:998-1042)
+//   hlr::hip::hip_arithmetic                   (tbb/arith.hh:1414-1510,
+//                                               arith/defaults.hh:20-104)
+// with y += alpha op(M) x, alpha == 0 a no-op, op in {apply_normal,
+// apply_conjugate, apply_transposed, apply_adjoint}, and errors thrown as
+// std::runtime_error (HLR_ASSERT/HLR_ERROR, utils/log.hh:40-67).  The matrix
+// is an hmvm plan on one HIP device; vectors are device arrays in the
+// internal (cluster) ordering.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hmvm.h"
+
+namespace hlr {
+namespace hip {
+
+enum matop_t { apply_normal = HMVM_OP_NORMAL, apply_transposed = HMVM_OP_TRANSPOSED,
+               apply_adjoint = HMVM_OP_ADJOINT, apply_conjugate = HMVM_OP_CONJUGATE };
+
+inline void check(int rc, const char* what) {
+    if (rc != HMVM_OK) throw std::runtime_error(std::string(what) + ": " + hmvm_error_string(rc));
+}
+
+inline void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// device-resident vector of value_t (double or float) of length n, possibly
+// with several columns (nrhs) of leading dimension n
+template <typename value_t>
+class device_vector {
+public:
+    device_vector() = default;
+    explicit device_vector(size_t n, int ncols = 1) : n_(n), ncols_(ncols) {
+        check_hip(hipMalloc(&p_, sizeof(value_t) * n * size_t(ncols)), "hipMalloc");
+        check_hip(hipMemset(p_, 0, sizeof(value_t) * n * size_t(ncols)), "hipMemset");
+    }
+    device_vector(const device_vector&) = delete;
+    device_vector& operator=(const device_vector&) = delete;
+    device_vector(device_vector&& o) noexcept { swap(o); }
+    device_vector& operator=(device_vector&& o) noexcept { swap(o); return *this; }
+    ~device_vector() { if (p_) (void)hipFree(p_); }
+
+    void copy_from(const std::vector<value_t>& h) {
+        if (h.size() != n_ * size_t(ncols_)) throw std::runtime_error("device_vector: size mismatch");
+        check_hip(hipMemcpy(p_, h.data(), sizeof(value_t) * h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    std::vector<value_t> to_host() const {
+        std::vector<value_t> h(n_ * size_t(ncols_));
+        check_hip(hipMemcpy(h.data(), p_, sizeof(value_t) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+        return h;
+    }
+    void fill_zero() { check_hip(hipMemset(p_, 0, sizeof(value_t) * n_ * size_t(ncols_)), "hipMemset"); }
+
+    value_t* data() { return p_; }
+    const value_t* data() const { return p_; }
+    size_t length() const { return n_; }
+    int ncols() const { return ncols_; }
+
+private:
+    void swap(device_vector& o) { std::swap(p_, o.p_); std::swap(n_, o.n_); std::swap(ncols_, o.ncols_); }
+    value_t* p_ = nullptr;
+    size_t n_ = 0;
+    int ncols_ = 1;
+};
+
+// H- or TLR-matrix held as a device plan
+class hmatrix {
+public:
+    hmatrix(const std::vector<hmvm_block>& blocks, const std::vector<double>& arena, int64_t nrows,
+            int64_t ncols, const hmvm_plan_opts* opts = nullptr, int device = 0) {
+        check(hmvm_plan_create(blocks.data(), int64_t(blocks.size()), nrows, ncols, arena.data(),
+                               int64_t(arena.size()), opts, device, &p_),
+              "hmvm_plan_create");
+        check(hmvm_plan_get_info(p_, &info_), "hmvm_plan_get_info");
+    }
+    hmatrix(const hmatrix&) = delete;
+    hmatrix& operator=(const hmatrix&) = delete;
+    ~hmatrix() { if (p_) hmvm_plan_destroy(p_); }
+
+    const hmvm_plan* plan() const { return p_; }
+    int64_t nrows() const { return info_.nrows; }
+    int64_t ncols() const { return info_.ncols; }
+    int64_t nrows(matop_t op) const { return (op == apply_transposed || op == apply_adjoint) ? ncols() : nrows(); }
+    int64_t ncols(matop_t op) const { return (op == apply_transposed || op == apply_adjoint) ? nrows() : ncols(); }
+    const hmvm_plan_info& info() const { return info_; }
+    // mul_vec_flops / mul_vec_datasize (arith/mulvec.hh:795-874)
+    double flops() const { return info_.flops; }
+    double data_byte_size() const { return info_.datasize; }
+
+private:
+    hmvm_plan* p_ = nullptr;
+    hmvm_plan_info info_{};
+};
+
+template <typename value_t> constexpr int vec_dtype();
+template <> constexpr int vec_dtype<double>() { return HMVM_F64; }
+template <> constexpr int vec_dtype<float>() { return HMVM_F32; }
+
+// y += alpha op(M) x
+template <typename value_t>
+void mul_vec(const value_t alpha, const matop_t op, const hmatrix& M, const device_vector<value_t>& x,
+             device_vector<value_t>& y, hipStream_t stream = nullptr) {
+    if (int64_t(x.length()) != M.ncols(op) || int64_t(y.length()) != M.nrows(op) || x.ncols() != y.ncols())
+        throw std::runtime_error("mul_vec: vector sizes do not match the matrix");
+    check(hmvm_apply(M.plan(), double(alpha), int(op), x.data(), int64_t(x.length()), y.data(),
+                     int64_t(y.length()), x.ncols(), vec_dtype<value_t>(), stream),
+          "hmvm_apply");
+}
+
+namespace tlr {
+
+// y += alpha op(M) x for a one-level (tile-low-rank) matrix
+template <typename value_t>
+void mul_vec(const value_t alpha, const matop_t op, const hmatrix& M, const device_vector<value_t>& x,
+             device_vector<value_t>& y, hipStream_t stream = nullptr) {
+    if (int64_t(x.length()) != M.ncols(op) || int64_t(y.length()) != M.nrows(op))
+        throw std::runtime_error("tlr::mul_vec: vector sizes do not match the matrix");
+    check(hmvm_tlr_apply(M.plan(), double(alpha), int(op), x.data(), y.data(), vec_dtype<value_t>(), stream),
+          "hmvm_tlr_apply");
+}
+
+} // namespace tlr
+
+// arithmetic object for generic algorithms (norm::spectral and friends call
+// arithmetic.prod(alpha, op, A, x, y), arith/detail/norm.hh:492-494)
+struct hip_arithmetic {
+    static constexpr bool is_arithmetic = true;
+    hipStream_t stream = nullptr;
+
+    template <typename value_t>
+    void mul_vec(const value_t alpha, const matop_t op, const hmatrix& M, const device_vector<value_t>& x,
+                 device_vector<value_t>& y) const {
+        hlr::hip::mul_vec(alpha, op, M, x, y, stream);
+    }
+
+    template <typename value_t>
+    void prod(const value_t alpha, const matop_t op, const hmatrix& M, const device_vector<value_t>& x,
+              device_vector<value_t>& y) const {
+        hlr::hip::mul_vec(alpha, op, M, x, y, stream);
+    }
+};
+
+} // namespace hip
+} // namespace hlr

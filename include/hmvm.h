@@ -230,14 +230,22 @@ int hmvm_plan_unpack_block(const hmvm_plan* plan, int64_t block,
                            double* a, double* b);
 
 /* ------------------------------------------------------------------------ */
-/* batching: many plans, many right-hand sides, one persistent launch       */
+/* batching: many plans, many right-hand sides, one launch sequence         */
 /* ------------------------------------------------------------------------ */
 
 int hmvm_batch_create(const hmvm_plan* const* plans, int nplans,
                       const int* nrhs, int vec_dtype, hmvm_batch** out);
 int hmvm_batch_destroy(hmvm_batch* batch);
 
-/* y_i += alpha_i A_i x_i for every plan i, in one persistent launch */
+/*
+ * y_i += alpha_i A_i x_i for every plan i (forward products; nrhs[i] in
+ * 1..64 right-hand sides, column-major with ldx[i] / ldy[i]).  The work of
+ * all members runs as six stream-ordered kernels (column-panel pass for one
+ * and for several right-hand sides, two partial-sum stages, row-panel pass
+ * for one and for several right-hand sides), one unit of work per
+ * workgroup, whatever the number of members.  No counterpart in the
+ * reference (it applies one matrix at a time, programs/mvm/mvm.hh:151-204).
+ */
 int hmvm_batch_apply(hmvm_batch* batch, const double* alphas,
                      const void* const* xs, const int64_t* ldx,
                      void* const* ys, const int64_t* ldy, void* stream);
@@ -248,10 +256,19 @@ int hmvm_batch_capture(hmvm_batch* batch, const double* alphas,
                        void* const* ys, const int64_t* ldy, void* stream);
 int hmvm_batch_replay(hmvm_batch* batch, void* stream);
 
-/* time candidate launch shapes on `stream` and keep the fastest */
+/*
+ * In-project autotuner: times the candidate schedules (unit order {member
+ * order, largest unit first} x partial-sum items per workgroup {1024, 4096})
+ * on `stream` with the given vectors (y accumulates alpha = 1 products, the
+ * caller resets it) and keeps the fastest; *chosen receives
+ * 2 * order + index of the item count.
+ */
 int hmvm_batch_tune(hmvm_batch* batch, const void* const* xs,
                     const int64_t* ldx, void* const* ys,
                     const int64_t* ldy, void* stream, int* chosen);
+
+/* synchronise `stream` and report a device error of the batch's launches */
+int hmvm_batch_check(hmvm_batch* batch, void* stream);
 
 const char* hmvm_error_string(int code);
 
