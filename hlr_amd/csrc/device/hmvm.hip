@@ -641,7 +641,7 @@ int hmvm_plan_get_info(const hmvm_plan* p, hmvm_plan_info* info) {
         fb += L.bytes_by_format[f];
     }
     info->factor_bytes = fb;
-    info->scale_bytes = int64_t(L.zscale.size() + L.vscale.size()) * 4;
+    info->scale_bytes = L.scale_bytes;
     info->flops = L.flops;
     info->datasize = L.datasize + 2.0 * 8.0 * double(L.nrows + L.ncols);
     info->one_level = L.one_level ? 1 : 0;

@@ -646,8 +646,18 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             }
         }
     }
-    // per-column scales count as data (4 bytes per stored panel column)
-    if (!L.zscale.empty()) { br += 4.0 * double(L.zscale.size()); bv += 4.0 * double(L.vscale.size()); }
+    // the scale of an FP8 panel column counts as data (4 bytes per column of
+    // an FP8 chunk or segment, which the kernels read; other columns' table
+    // entries are never read)
+    int64_t s8r = 0, s8v = 0;
+    for (const RChunk& c : L.rchunks)
+        if (c.fmt == FP8) s8r += int64_t(c.ngroups) * fmt_width(FP8);
+    for (const VItem& v : L.vitems)
+        if (v.fmt == FP8) s8v += int64_t(v.ng) * fmt_width(FP8);
+    L.scale_bytes = 4 * (s8r + s8v);
+    br += 4.0 * double(s8r);
+    bv += 4.0 * double(s8v);
+    ds += double(L.scale_bytes);
     L.bytes_rpass = br;
     L.bytes_vpass = bv;
     L.flops = fl;

@@ -57,6 +57,7 @@ struct HostLayout {
     // statistics
     int64_t ndense = 0, nlowrank = 0;
     int64_t bytes_by_format[NFMT] = {0, 0, 0, 0};
+    int64_t scale_bytes = 0;          // FP8 column scales read by the kernels
     int64_t cols_by_format[NFMT] = {0, 0, 0, 0};
     double flops = 0, datasize = 0, bytes_rpass = 0, bytes_vpass = 0;
     int32_t max_vitem_rows = 0;

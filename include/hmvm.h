@@ -118,7 +118,7 @@ typedef struct hmvm_plan_info {
     int64_t factor_bytes;        /* device bytes of all stored factors           */
     int64_t bytes_by_format[4];  /* FP64, FP32, BF16, FP8 factor bytes           */
     int64_t cols_by_format[4];   /* low-rank columns stored in each format       */
-    int64_t scale_bytes;         /* per-column scale factors                     */
+    int64_t scale_bytes;         /* FP8 column scales (4 B per FP8 panel column) */
     double  flops;               /* mul_vec_flops for nrhs = 1                   */
     double  datasize;            /* mul_vec_datasize restated for this storage   */
     int64_t index_bytes;         /* per-column source/destination index tables   */
