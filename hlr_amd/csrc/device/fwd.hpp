@@ -21,7 +21,7 @@ template <> struct IsFp8<uint8_t> { static constexpr bool value = true; };
 
 template <typename T, typename XT>
 __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp, const XT* __restrict__ x,
-                                 double* __restrict__ Pb, double* xsraw, int lane) {
+                                 double* __restrict__ Pb, void* xsraw, int lane) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
@@ -29,7 +29,7 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     // x slices of the segments, row stride kXsStride (odd): the lanes of
     // different segments read different rows i of their own slice in the
     // same step, which then fall into different LDS banks
-    A* xs = reinterpret_cast<A*>(xsraw);            // [kPackSegs][kXsStride]
+    A* xs = static_cast<A*>(xsraw);                 // [kPackSegs][kXsStride]
     const int nseg = pk.nseg;
     for (int idx = lane; idx < nseg * 64; idx += 64) {
         const int j = idx >> 6, i = idx & 63;

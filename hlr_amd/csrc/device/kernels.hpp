@@ -56,10 +56,21 @@ struct DevPlan {
 
 template <typename T> struct Elem;
 
+// raw(): the 16-byte non-temporal load alone; decode(): its W values in the
+// accumulation type; load() = decode(raw()).  Splitting the two lets a loop
+// keep several raw loads in flight while it decodes earlier ones.
+__device__ inline u32x4 raw16(const void* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+}
+
 template <> struct Elem<double> {
     static constexpr int F = F64;
     static constexpr int W = 2;
     using Acc = double;
+    __device__ static inline void decode(u32x4 v, double* out) {
+        out[0] = __hiloint2double(int(v.y), int(v.x));
+        out[1] = __hiloint2double(int(v.w), int(v.z));
+    }
     __device__ static inline void load(const double* p, double* out) {
         const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p));
         out[0] = v.x; out[1] = v.y;
@@ -70,6 +81,10 @@ template <> struct Elem<float> {
     static constexpr int F = F32;
     static constexpr int W = 4;
     using Acc = float;
+    __device__ static inline void decode(u32x4 v, float* out) {
+        out[0] = __uint_as_float(v.x); out[1] = __uint_as_float(v.y);
+        out[2] = __uint_as_float(v.z); out[3] = __uint_as_float(v.w);
+    }
     __device__ static inline void load(const float* p, float* out) {
         const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
         out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
@@ -80,8 +95,7 @@ template <> struct Elem<uint16_t> {   // BF16
     static constexpr int F = BF16;
     static constexpr int W = 8;
     using Acc = float;
-    __device__ static inline void load(const uint16_t* p, float* out) {
-        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    __device__ static inline void decode(u32x4 v, float* out) {
         const unsigned w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -89,14 +103,14 @@ template <> struct Elem<uint16_t> {   // BF16
             out[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
         }
     }
+    __device__ static inline void load(const uint16_t* p, float* out) { decode(raw16(p), out); }
 };
 
 template <> struct Elem<uint8_t> {    // FP8 E4M3 (OCP)
     static constexpr int F = FP8;
     static constexpr int W = 16;
     using Acc = float;
-    __device__ static inline void load(const uint8_t* p, float* out) {
-        const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    __device__ static inline void decode(u32x4 v, float* out) {
         const unsigned w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -105,6 +119,7 @@ template <> struct Elem<uint8_t> {    // FP8 E4M3 (OCP)
             out[4 * i] = lo.x; out[4 * i + 1] = lo.y; out[4 * i + 2] = hi.x; out[4 * i + 3] = hi.y;
         }
     }
+    __device__ static inline void load(const uint8_t* p, float* out) { decode(raw16(p), out); }
 };
 
 __device__ inline void wave_sync() {

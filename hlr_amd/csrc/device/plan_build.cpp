@@ -143,9 +143,17 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     L.ncols = ncols;
     L.precision = opts.precision;
     L.eps = opts.eps;
+    // dense leaves under VALR: as requested, or (dense_prec == VALR) by the
+    // same rate rule as the columns, restricted to FP32 / FP64: FP32 while
+    // eps needs at most its 23 mantissa bits
+    if (opts.precision == HMVM_PREC_VALR && opts.dense_prec != HMVM_PREC_FP64 &&
+        opts.dense_prec != HMVM_PREC_FP32 && opts.dense_prec != HMVM_PREC_VALR)
+        return HMVM_ERR_INVALID_ARG;
     L.dense_fmt = (opts.precision == HMVM_PREC_FP64) ? F64
                 : (opts.precision == HMVM_PREC_FP32) ? F32
-                : (opts.dense_prec == HMVM_PREC_FP64 ? F64 : F32);
+                : (opts.dense_prec == HMVM_PREC_FP64) ? F64
+                : (opts.dense_prec == HMVM_PREC_FP32) ? F32
+                : (eps_to_rate(opts.eps) <= 23 ? F32 : F64);
     L.blocks.assign(blocks, blocks + nb);
 
     // ---- validate leaves (cf. HLR_ASSERT in apply_add, check_data()) ----

@@ -70,6 +70,7 @@ int main(int argc, char** argv) {
         hmvm_plan_opts opts{};
         opts.precision = prec == "fp32" ? HMVM_PREC_FP32 : prec == "valr" ? HMVM_PREC_VALR : HMVM_PREC_FP64;
         opts.eps = cfg.eps;
+        opts.dense_prec = HMVM_PREC_VALR;          // dense leaves FP32 unless eps needs FP64
         opts.check_finite = 1;
         hmatrix A(blocks, arena, n, n, &opts, 0);
         std::printf("plan      : %s, %.3f GB of factors, %lld dense / %lld low-rank leaves\n", prec.c_str(),

@@ -42,14 +42,16 @@ class Plan:
     """
 
     def __init__(self, blocks: np.ndarray, arena: np.ndarray, nrows: int, ncols: int,
-                 precision: str = "fp64", eps: float = 0.0, dense_prec: str = "fp32",
+                 precision: str = "fp64", eps: float = 0.0, dense_prec: str = "auto",
                  device: int = 0, check_finite: bool = True, force_format: str | None = None):
         self._p = None
         blocks = np.ascontiguousarray(blocks, dtype=N.block_dtype())
         arena = np.ascontiguousarray(arena, dtype=np.float64)
         opts = N.HmvmPlanOpts()
         opts.precision = PRECISIONS[precision]
-        opts.dense_prec = N.PREC_FP64 if dense_prec == "fp64" else N.PREC_FP32
+        # dense leaves under "valr": "fp64", "fp32", or "auto" (FP32 while eps
+        # needs at most 23 mantissa bits, else FP64)
+        opts.dense_prec = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "auto": N.PREC_VALR}[dense_prec]
         opts.eps = float(eps)
         opts.check_finite = 1 if check_finite else 0
         opts.force_format = 0 if force_format is None else 1 + FORMAT_NAMES.index(force_format)

@@ -102,8 +102,9 @@ enum {
 
 typedef struct hmvm_plan_opts {
     int32_t precision;    /* HMVM_PREC_*                                          */
-    int32_t dense_prec;   /* HMVM_PREC_FP64 or HMVM_PREC_FP32 for dense leaves
-                             under HMVM_PREC_VALR (default FP32)                  */
+    int32_t dense_prec;   /* dense leaves under HMVM_PREC_VALR: HMVM_PREC_FP64,
+                             HMVM_PREC_FP32, or HMVM_PREC_VALR = by eps (FP32 while
+                             ceil(-log2 eps) <= 23, else FP64)                   */
     double  eps;          /* compression tolerance for HMVM_PREC_VALR             */
     int32_t check_finite; /* reject NaN/Inf factors at creation (default 1)       */
     int32_t force_format; /* 0: formats from `precision`; 1 + HMVM_FMT_*: store

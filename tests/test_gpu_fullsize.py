@@ -8,7 +8,8 @@
     of L = 1/2 ||A x - b||^2 with respect to x against the oracle's
     transposed product on the stored factors, and the gradients with
     respect to x and all factors against central finite differences
-    (h = 1e-3, 32 random directions)
+    (h = 1e-3, 32 random directions; the bar adds the measured rounding noise
+    of the FP32-accumulated loss, divided by h)
 Each prints its measured errors."""
 from __future__ import annotations
 
@@ -108,15 +109,27 @@ def test_tlr_gradients(cuda):
     deq = dequantized_arena(plan, hm)
     gref = np.zeros(hm.n)
     oracle.mul_vec(hm.blocks, deq, 1.0, N.OP_TRANSPOSED, r.cpu().numpy(), gref)
-    del deq
     eg = rel(x.grad.cpu().numpy(), gref)
     print(f"[gpu] TLR n={hm.n} dL/dx vs oracle transposed rel.err={eg:.3e}", flush=True)
     assert eg <= 2e-6
+    # rounding noise of one loss evaluation: the GPU loss (FP32 accumulation
+    # inside tiles) against the same loss in FP64 on the stored factors
+    xh = x.detach().cpu().numpy()
+    ax_ref = np.zeros(hm.n)
+    oracle.mul_vec(hm.blocks, deq, 1.0, N.OP_NORMAL, xh, ax_ref)
+    del deq
+    L_ref = 0.5 * float(np.dot(ax_ref - b.cpu().numpy(), ax_ref - b.cpu().numpy()))
+    dL = abs(float(L) - L_ref)
+    print(f"[gpu] TLR n={hm.n} loss rounding |L_gpu - L_fp64| = {dL:.3e} (L = {L_ref:.6e})", flush=True)
 
     # central finite differences, h = 1e-3, 32 random directions in (x, factors);
-    # directions drawn on the device (the factor arena has ~1.7e9 entries)
+    # directions drawn on the device (the factor arena has ~1.7e9 entries).
+    # L is quadratic in (x, factors), so central differences have no
+    # truncation error: what remains is the rounding of the two loss values,
+    # at most about 2 dL / (2 h), allowed on top of the relative bar.
     gen = torch.Generator(device=cuda).manual_seed(11)
     h = 1e-3
+    noise = 4.0 * dL / h
     worst = 0.0
     with torch.no_grad():
         Ud = U0.double()
@@ -128,7 +141,9 @@ def test_tlr_gradients(cuda):
             fd = (lp - lm) / (2 * h)
             ax = float(torch.dot(x.grad, dx))
             au = float(torch.dot(Ua.grad.double(), du))
-            worst = max(worst, abs(fd - (ax + au)) / (abs(ax) + abs(au)))
+            dev = abs(fd - (ax + au))
+            worst = max(worst, dev / (abs(ax) + abs(au)))
+            assert dev <= 1e-3 * (abs(ax) + abs(au)) + noise, (fd, ax, au, noise)
             del du
-    print(f"[gpu] TLR n={hm.n} worst FD relative deviation {worst:.2e}", flush=True)
-    assert worst <= 1e-3
+    print(f"[gpu] TLR n={hm.n} worst FD relative deviation {worst:.2e} "
+          f"(rounding allowance {noise:.2e} absolute)", flush=True)

@@ -176,3 +176,28 @@ def test_apply_on_host_plan_rejected():
     rc = N.hmvm().hmvm_apply(None, 1.0, 0, None, 0, None, 0, 1, N.F64, None)
     assert rc == 1
     assert N.error_string(3) == "leaf blocks overlap"
+
+
+@pytest.mark.parametrize("eps,exact", [(1e-4, False), (1e-6, False), (1e-8, True)])
+def test_dense_precision_follows_eps(sphere, eps, exact):
+    """Dense leaves under the default rule: FP32 while ceil(-log2 eps) <= 23,
+    FP64 below (FP32 rounding alone would exceed eps)."""
+    p = Plan.from_hmatrix(sphere, precision="valr", eps=eps, device=-1)
+    ar = [p.arena_numpy(f) for f in range(4)]
+    assert (oracle.eps_to_rate(eps) > 23) == exact
+    for b, blk in enumerate(sphere.blocks[:300]):
+        if int(blk["kind"]) != N.BLOCK_DENSE:
+            continue
+        m, n = int(blk["nrows"]), int(blk["ncols"])
+        D, _ = p.unpack_block(b, ar)
+        ref = sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T
+        assert np.array_equal(D, ref if exact else ref.astype(np.float32).astype(np.float64))
+    # an explicit request overrides the rule
+    q = Plan.from_hmatrix(sphere, precision="valr", eps=eps, dense_prec="fp32", device=-1)
+    qa = [q.arena_numpy(f) for f in range(4)]
+    b = next(i for i, blk in enumerate(sphere.blocks) if int(blk["kind"]) == N.BLOCK_DENSE)
+    D, _ = q.unpack_block(b, qa)
+    blk = sphere.blocks[b]
+    m, n = int(blk["nrows"]), int(blk["ncols"])
+    ref = sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T
+    assert np.array_equal(D, ref.astype(np.float32).astype(np.float64))
