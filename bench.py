@@ -48,6 +48,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nrhs", type=int, default=1, help="right-hand sides per call")
+    ap.add_argument("--op", choices=["normal", "transposed"], default="normal",
+                    help="product timed (the square sphere matrix keeps the same sizes)")
     ap.add_argument("--precision", choices=["both", "fp64", "valr"], default="both",
                     help="time only one plan (profiling runs); the JSON line needs both")
     ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
@@ -79,34 +81,38 @@ def main() -> int:
         f"built in {time.time() - t0:.1f} s")
 
     R = args.nrhs
+    OP = N.OP_TRANSPOSED if args.op == "transposed" else N.OP_NORMAL
+    if OP != N.OP_NORMAL and R > 1:
+        log("transposed products take one right-hand side per call here")
+        return 2
     x_h = random_vector(hm.n, 2) if R == 1 else np.stack([random_vector(hm.n, 2 + j) for j in range(R)])
     x = torch.as_tensor(x_h, device=dev)
     stream = torch.cuda.current_stream()
 
     def time_plan(plan, steps, warmup):
         # workspace allocated once (hmvm_apply_ws): the timed loop is kernels only
-        ws = torch.empty(plan.workspace_bytes(N.OP_NORMAL, R) // 8 + 1, dtype=torch.float64, device=dev)
+        ws = torch.empty(plan.workspace_bytes(OP, R) // 8 + 1, dtype=torch.float64, device=dev)
         y = torch.zeros_like(x)
         for _ in range(warmup):
-            plan.apply_ws(2.0, N.OP_NORMAL, x, y, ws)
+            plan.apply_ws(2.0, OP, x, y, ws)
         torch.cuda.synchronize()
         start = torch.cuda.Event(enable_timing=True)
         stop = torch.cuda.Event(enable_timing=True)
         start.record(stream)
         for _ in range(steps):
-            plan.apply_ws(2.0, N.OP_NORMAL, x, y, ws)
+            plan.apply_ws(2.0, OP, x, y, ws)
         stop.record(stream)
         torch.cuda.synchronize()
         ms = start.elapsed_time(stop) / steps
         if R > 1:
             return ms, np.zeros(3)
         # per-phase durations (HIP events on the launch stream)
-        phases = np.array([plan.apply_timed(2.0, N.OP_NORMAL, x, y) for _ in range(max(5, steps // 10))])
+        phases = np.array([plan.apply_timed(2.0, OP, x, y) for _ in range(max(5, steps // 10))])
         return ms, phases.mean(axis=0)
 
     def result(plan):
         y = torch.zeros_like(x)
-        plan.apply(1.0, N.OP_NORMAL, x, y)
+        plan.apply(1.0, OP, x, y)
         torch.cuda.synchronize()
         return y.cpu().numpy()
 
@@ -165,7 +171,7 @@ def main() -> int:
     d = (ymp - y64).reshape(R, -1)
     err_gpu = float(max(np.linalg.norm(d[j]) / np.linalg.norm(y64.reshape(R, -1)[j]) for j in range(R)))
     cpu = None
-    if not args.no_cpu and R == 1:
+    if not args.no_cpu and R == 1 and OP == N.OP_NORMAL:
         # cpu_baseline leg: the oracle's restatement of tbb::mul_vec_chunk, timed
         import oracle
         cores = min(len(os.sched_getaffinity(0)), 16)
@@ -181,11 +187,16 @@ def main() -> int:
     # ---- roofline of the dominant kernel: the row-panel pass (k_ufwd) ----
     n = hm.n
     mvm_bytes = imp["datasize"] + 8.0 * 2 * (R - 1) * (n + n)    # vectors counted per right-hand side
-    if R == 1:
+    if R == 1 and OP == N.OP_NORMAL:
         vec_bytes = 8.0 * (n + n)               # read x, read+write y once per pass
         r_bytes = imp["bytes_rpass"] + vec_bytes
         achieved = r_bytes / (phmp[2] * 1e-3) / 1e9
         rkernel = "k_ufwd (row-panel pass, mixed precision)"
+    elif R == 1:
+        # transposed: the row-panel pass is the first phase and reads x once
+        r_bytes = imp["bytes_rpass"] + 8.0 * n
+        achieved = r_bytes / (phmp[0] * 1e-3) / 1e9
+        rkernel = "k_rtrn (row-panel pass of the transposed product, mixed precision)"
     else:
         # whole call: the matrix-core kernels stream the factors once per call
         r_bytes = mvm_bytes
@@ -198,7 +209,7 @@ def main() -> int:
     # (FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated), when it exists
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_invr_sphere_n{n}.txt"
-    if tfile.exists() and args.eps == 1e-6 and R == 1:
+    if tfile.exists() and args.eps == 1e-6 and R == 1 and OP == N.OP_NORMAL:
         for line in tfile.read_text().splitlines():
             f = line.split()
             if len(f) >= 4 and not line.startswith("#") and f[0] == "valr" and f[1].startswith("k_ufwd"):
@@ -217,7 +228,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32-acc mixed fp32/bf16/fp8-e4m3 storage",
         "data": "synthetic (seeded points, generated H-matrix)",
-        "config": {"workload": f"1/r on unit sphere, n={n}, leaf 64, eta 2, ACA eps={args.eps:g}, nrhs {R}",
+        "config": {"workload": f"1/r on unit sphere, n={n}, leaf 64, eta 2, ACA eps={args.eps:g}, nrhs {R}"
+                               + ("" if OP == N.OP_NORMAL else ", transposed product"),
                    "precision": "valr", "parallelism": "single GPU"},
         "fp64_mvm_per_s": 1000.0 * R / ms64,
         "fp64_ms_per_step": ms64,

@@ -138,13 +138,21 @@ __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, cons
 // transposed: R pass with column reduction, one wave per chunk
 // ---------------------------------------------------------------------------
 
+// Each step loads kRtLoads groups (raw 16-byte loads, all in flight
+// together), then reduces them across the lanes K columns at a time.
+constexpr int kRtLoads = 8;
+
 template <typename T, typename XT>
 __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __restrict__ ids, int64_t n,
                                                    const XT* __restrict__ x, double* __restrict__ Q) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
-    constexpr int SG = 64 / W;   // groups per 64 columns
+    constexpr int K = (sizeof(A) == 8) ? 8 : 16;    // columns reduced together (16 registers live)
+    constexpr int SG = K / W;                       // groups per reduction
+    constexpr int NL = kRtLoads;                    // groups per step
+    constexpr int LPC = 64 / K;                     // lanes holding one column sum
+    static_assert(SG >= 1 && NL % SG == 0, "a reduction covers whole groups, a step whole reductions");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
     if (idx >= n) return;
@@ -152,31 +160,35 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __r
     const RowPanel R = P.rp[C.panel];
     const int m = R.m;
     const A xi = (lane < m) ? A(x[R.row0 + lane]) : A(0);
-    const T* base = static_cast<const T*>(P.arena[E::F]) + C.data_off + int64_t(lane) * W;
+    const bool on = lane < m;
+    const T* base = static_cast<const T*>(P.arena[E::F]) + C.data_off + int64_t(on ? lane : 0) * W;
     const int64_t gs = int64_t(m) * W;
-    for (int g0 = 0; g0 < C.ngroups; g0 += SG) {
-        A v[64];
+    const int ncol = C.ngroups * W;
+#pragma unroll 1
+    for (int g0 = 0; g0 < C.ngroups; g0 += NL) {
+        u32x4 raw[NL];
 #pragma unroll
-        for (int gg = 0; gg < SG; ++gg) {
-            const int g = g0 + gg;
-            if (g < C.ngroups && lane < m) {
+        for (int u = 0; u < NL; ++u)
+            raw[u] = (g0 + u < C.ngroups) ? raw16(base + (g0 + u) * gs) : u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int sb = 0; sb < NL / SG; ++sb) {
+            A v[K];
+#pragma unroll
+            for (int gg = 0; gg < SG; ++gg) {
                 A u[W];
-                E::load(base + g * gs, u);
+                E::decode(raw[sb * SG + gg], u);
 #pragma unroll
-                for (int w = 0; w < W; ++w) v[gg * W + w] = u[w] * xi;
-            } else {
-#pragma unroll
-                for (int w = 0; w < W; ++w) v[gg * W + w] = A(0);
+                for (int w = 0; w < W; ++w) v[gg * W + w] = u[w] * xi;     // xi = 0 on rows past m
             }
-        }
-        const A s = butterfly64(v, lane);
-        const int col = g0 * W + lane;
-        if (col < C.ngroups * W) {
-            const int64_t gc = C.col0 + col;
-            const int32_t d = P.rdest[gc];
-            double sv = double(s);
-            if (IsFp8<T>::value) sv *= double(P.zscale[gc]);
-            if (d >= 0) Q[d] = sv;
+            const A s = xreduce<K>(v, lane);
+            const int col = (g0 + sb * SG) * W + lane / LPC;
+            if ((lane % LPC) == 0 && col < ncol) {
+                const int64_t gc = C.col0 + col;
+                const int32_t d = P.rdest[gc];
+                double sv = double(s);
+                if (IsFp8<T>::value) sv *= double(P.zscale[gc]);
+                if (d >= 0) Q[d] = sv;
+            }
         }
     }
 }
@@ -185,34 +197,58 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __r
 // transposed: V pass, y_q += V_q s  (+ dense partials), workgroup per panel
 // ---------------------------------------------------------------------------
 
+constexpr int kTRows = 4;           // rows reduced together in the transposed V pass
+constexpr int kVtLoads = 8;         // rows loaded per step of the transposed V pass
+
+// lane = group of the item; rows in blocks of kTRows: each lane forms its
+// rows' dot products with s, the wave reduces them across the groups, and
+// the lanes L with L % (64 / kTRows) == 0 add the sum of row
+// kTRows b + L / (64 / kTRows) to the wave's own row sums in LDS (rw, 64
+// entries)
 template <typename T>
 __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VItem& it,
-                                 const double* __restrict__ s, double (&acc)[64], int lane) {
+                                 const double* __restrict__ s, double* rw, int lane) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
-    if (lane >= it.ng) return;
-    const int g = lane;
+    const bool on = lane < it.ng;
+    const int g = on ? lane : 0;
     double sv[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const int64_t vc = C.vcol0[E::F] + int64_t(it.g0 + g) * W + w;
-        const int32_t e = P.vsrc[vc];
+        const int32_t e = on ? P.vsrc[vc] : -1;
         double v = (e >= 0) ? s[e] : 0.0;
-        if (IsFp8<T>::value) v *= double(P.vscale[vc]);
+        if (IsFp8<T>::value && on) v *= double(P.vscale[vc]);
         sv[w] = v;
     }
     const int64_t ld = int64_t(it.ng) * W;
     const T* base = static_cast<const T*>(P.arena[E::F]) + it.data_off + int64_t(g) * W;
+    constexpr int NL = kVtLoads;
+    static_assert(NL % kTRows == 0, "a step covers whole row blocks");
+#pragma unroll 1
+    for (int i0 = 0; i0 < C.m; i0 += NL) {
+        // rows of the step, all loads in flight together (lanes past the
+        // item's groups read group 0 and meet sv = 0; rows past m are not read)
+        u32x4 raw[NL];
 #pragma unroll
-    for (int i = 0; i < 64; ++i) {
-        if (i < C.m) {
-            A v[W];
-            E::load(base + int64_t(i) * ld, v);
-            double d = 0;
+        for (int r = 0; r < NL; ++r)
+            raw[r] = (i0 + r < C.m) ? raw16(base + int64_t(i0 + r) * ld) : u32x4{0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int w = 0; w < W; ++w) d += double(v[w]) * sv[w];
-            acc[i] += d;
+        for (int sb = 0; sb < NL / kTRows; ++sb) {
+            double d[kTRows];
+#pragma unroll
+            for (int r = 0; r < kTRows; ++r) {
+                A v[W];
+                E::decode(raw[sb * kTRows + r], v);
+                double acc = 0.0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) acc += double(v[w]) * sv[w];
+                d[r] = acc;
+            }
+            const double r = xreduce<kTRows>(d, lane);
+            const int row = i0 + sb * kTRows + lane / (64 / kTRows);
+            if ((lane % (64 / kTRows)) == 0 && row < 64) rw[row] += r;
         }
     }
 }
@@ -223,20 +259,16 @@ __global__ __launch_bounds__(kThreads) void k_vtrn(DevPlan P, double alpha, cons
     __shared__ double red[kWaves][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const ColPanel C = P.cp[blockIdx.x];
-    double acc[64];
-#pragma unroll
-    for (int i = 0; i < 64; ++i) acc[i] = 0;
+    red[wave][lane] = 0.0;          // the wave's row sums; only this wave touches its row
     for (int j = wave; j < C.nitems; j += kWaves) {
         const VItem it = P.vitems[C.item0 + j];
         switch (it.fmt) {
-        case F64: item_vtrn<double>(P, C, it, s, acc, lane); break;
-        case F32: item_vtrn<float>(P, C, it, s, acc, lane); break;
-        case BF16: item_vtrn<uint16_t>(P, C, it, s, acc, lane); break;
-        default: item_vtrn<uint8_t>(P, C, it, s, acc, lane); break;
+        case F64: item_vtrn<double>(P, C, it, s, red[wave], lane); break;
+        case F32: item_vtrn<float>(P, C, it, s, red[wave], lane); break;
+        case BF16: item_vtrn<uint16_t>(P, C, it, s, red[wave], lane); break;
+        default: item_vtrn<uint8_t>(P, C, it, s, red[wave], lane); break;
         }
     }
-    const double r = butterfly64(acc, lane);
-    red[wave][lane] = r;
     __syncthreads();
     if (wave == 0 && lane < C.m) {
         const int64_t j = C.col0 + lane;

@@ -144,4 +144,29 @@ __device__ inline A butterfly64(A (&v)[64], int lane) {
     return v[0];
 }
 
+// Sum v[k] over the 64 lanes for k = 0..K-1 (K a power of two <= 64):
+// log2(K) halving steps (each lane keeps one half and adds its partner's),
+// then log2(64 / K) plain steps.  Afterwards lane L holds the sum for
+// k = L / (64 / K); the 64 / K lanes of a group hold the same value.
+// About one shuffle per summed index for any K, with K values per lane
+// live instead of 64.  The order is fixed, so results are reproducible.
+template <int K, typename A>
+__device__ inline A xreduce(A (&v)[K], int lane) {
+    static_assert(K >= 1 && K <= 64 && (K & (K - 1)) == 0, "K must be a power of two <= 64");
+#pragma unroll
+    for (int s = 32, h = K / 2; h >= 1; s >>= 1, h >>= 1) {
+        const bool up = (lane & s) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            const A send = up ? v[j] : v[j + h];
+            const A keep = up ? v[j + h] : v[j];
+            v[j] = keep + __shfl_xor(send, s, 64);
+        }
+    }
+    A r = v[0];
+#pragma unroll
+    for (int s = 32 / K; s >= 1; s >>= 1) r += __shfl_xor(r, s, 64);
+    return r;
+}
+
 } // namespace hmvm
