@@ -286,15 +286,16 @@ def bench_batch(args) -> int:
     chosen = b.tune(xs, ys)
     stream = torch.cuda.current_stream()
 
-    def timed(fn, steps, warmup):
+    def timed(fn, steps, warmup, st=stream):
+        # events on the stream the work is issued on
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
+        e0.record(st)
         for _ in range(steps):
             fn()
-        e1.record(stream)
+        e1.record(st)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps
 
@@ -307,9 +308,14 @@ def bench_batch(args) -> int:
             p.apply_ws(1.0, N.OP_NORMAL, x, y, w)
 
     ms_loop = timed(per_plan, args.steps, args.warmup)
-    b.capture(xs, ys)
-    ms_graph = timed(lambda: b.replay(), args.steps, args.warmup)
-    b.check()
+    # graph capture needs a stream of its own (not the legacy default stream)
+    gs = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(gs):
+        b.capture(xs, ys, stream=gs)
+    gs.synchronize()
+    ms_graph = timed(lambda: b.replay(stream=gs), args.steps, args.warmup, st=gs)
+    b.check(stream=gs)
     products = float(sum(nrhs))
     # factor bytes once per member and call, vectors per right-hand side
     nbytes = float(sum(i["datasize"] + 8.0 * 2 * (r - 1) * (i["nrows"] + i["ncols"]) for i, r in zip(infos, nrhs)))

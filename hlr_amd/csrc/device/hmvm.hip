@@ -139,11 +139,13 @@ __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, cons
 // ---------------------------------------------------------------------------
 
 // Each step loads kRtLoads groups (raw 16-byte loads, all in flight
-// together), then reduces them across the lanes K columns at a time.
+// together), then reduces them across the lanes K columns at a time.  A
+// wave takes one work item: at most kRtSpan consecutive groups of a chunk
+// (every column belongs to one item, so each Q entry has one writer).
 constexpr int kRtLoads = 8;
 
 template <typename T, typename XT>
-__global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __restrict__ ids, int64_t n,
+__global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const RtItem* __restrict__ items, int64_t n,
                                                    const XT* __restrict__ x, double* __restrict__ Q) {
     using E = Elem<T>;
     using A = typename E::Acc;
@@ -156,7 +158,9 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __r
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
     if (idx >= n) return;
-    const RChunk C = P.rchunks[ids[idx]];
+    const RtItem wi = items[idx];
+    const RChunk C = P.rchunks[wi.chunk];
+    const int gend = min(wi.g0 + kRtSpan, int(C.ngroups));
     const RowPanel R = P.rp[C.panel];
     const int m = R.m;
     const A xi = (lane < m) ? A(x[R.row0 + lane]) : A(0);
@@ -165,11 +169,11 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __r
     const int64_t gs = int64_t(m) * W;
     const int ncol = C.ngroups * W;
 #pragma unroll 1
-    for (int g0 = 0; g0 < C.ngroups; g0 += NL) {
+    for (int g0 = wi.g0; g0 < gend; g0 += NL) {
         u32x4 raw[NL];
 #pragma unroll
         for (int u = 0; u < NL; ++u)
-            raw[u] = (g0 + u < C.ngroups) ? raw16(base + (g0 + u) * gs) : u32x4{0u, 0u, 0u, 0u};
+            raw[u] = (g0 + u < gend) ? raw16(base + (g0 + u) * gs) : u32x4{0u, 0u, 0u, 0u};
 #pragma unroll
         for (int sb = 0; sb < NL / SG; ++sb) {
             A v[K];
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const int32_t* __r
             }
             const A s = xreduce<K>(v, lane);
             const int col = (g0 + sb * SG) * W + lane / LPC;
-            if ((lane % LPC) == 0 && col < ncol) {
+            if ((lane % LPC) == 0 && col < min(ncol, gend * W)) {
                 const int64_t gc = C.col0 + col;
                 const int32_t d = P.rdest[gc];
                 double sv = double(s);
@@ -286,19 +290,21 @@ __global__ __launch_bounds__(kThreads) void k_vtrn(DevPlan P, double alpha, cons
 
 // dR(i, c) = ybar_i * z_c with z = x_j (dense) | t_e (low-rank)
 template <typename T, typename XT>
-__global__ __launch_bounds__(kThreads) void k_grad_r(DevPlan P, const int32_t* __restrict__ ids, int64_t n,
+__global__ __launch_bounds__(kThreads) void k_grad_r(DevPlan P, const RtItem* __restrict__ items, int64_t n,
                                                      const XT* __restrict__ x, const XT* __restrict__ ybar,
                                                      const double* __restrict__ t, double* __restrict__ gR) {
     constexpr int W = Elem<T>::W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
     if (idx >= n) return;
-    const RChunk C = P.rchunks[ids[idx]];
+    const RtItem wi = items[idx];
+    const RChunk C = P.rchunks[wi.chunk];
+    const int gend = min(wi.g0 + kRtSpan, int(C.ngroups));
     const RowPanel R = P.rp[C.panel];
     const int m = R.m;
     if (lane >= m) return;
     const double yb = double(ybar[R.row0 + lane]);
-    for (int g = 0; g < C.ngroups; ++g) {
+    for (int g = wi.g0; g < gend; ++g) {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             const int64_t gc = C.col0 + int64_t(g) * W + w;
@@ -347,9 +353,10 @@ struct hmvm_plan {
     DevPlan dp{};
     std::vector<void*> allocs;
     void* arena_dev[NFMT] = {nullptr, nullptr, nullptr, nullptr};
-    // R chunk ids grouped by format (transposed and gradient launches)
-    int32_t* rchunk_ids_dev = nullptr;
-    int64_t rchunk_start[NFMT + 1] = {0, 0, 0, 0, 0};
+    // R work items (chunk, first group; at most kRtSpan groups) grouped by
+    // format, for the transposed and gradient launches
+    const RtItem* rtitems_dev = nullptr;
+    int64_t rtitem_start[NFMT + 1] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -411,15 +418,15 @@ int upload_plan(hmvm_plan* p) {
     UP(qs_pos0, L.qs_pos0);
     UP(qs_nseg, L.qs_nseg);
 #undef UP
-    std::vector<int32_t> ids;
+    std::vector<RtItem> items;
     for (int f = 0; f < NFMT; ++f) {
-        p->rchunk_start[f] = int64_t(ids.size());
-        for (size_t c = 0; c < L.rchunks.size(); ++c) if (L.rchunks[c].fmt == f) ids.push_back(int32_t(c));
+        p->rtitem_start[f] = int64_t(items.size());
+        for (size_t c = 0; c < L.rchunks.size(); ++c)
+            if (L.rchunks[c].fmt == f)
+                for (int32_t g0 = 0; g0 < L.rchunks[c].ngroups; g0 += kRtSpan) items.push_back({int32_t(c), g0});
     }
-    p->rchunk_start[NFMT] = int64_t(ids.size());
-    const int32_t* idd = nullptr;
-    if ((rc = upload(p, ids, &idd)) != HMVM_OK) return rc;
-    p->rchunk_ids_dev = const_cast<int32_t*>(idd);
+    p->rtitem_start[NFMT] = int64_t(items.size());
+    if ((rc = upload(p, items, &p->rtitems_dev)) != HMVM_OK) return rc;
     // arenas
     const void* a[NFMT] = {nullptr, nullptr, nullptr, nullptr};
     if ((rc = upload(p, L.a64, reinterpret_cast<const double**>(&a[0]))) != HMVM_OK) return rc;
@@ -501,9 +508,9 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
         if (ev) (void)hipEventRecord(ev[3], st);
     } else {
         for (int f = 0; f < NFMT; ++f) {
-            const int64_t n = p->rchunk_start[f + 1] - p->rchunk_start[f];
+            const int64_t n = p->rtitem_start[f + 1] - p->rtitem_start[f];
             if (n == 0) continue;
-            const int32_t* ids = p->rchunk_ids_dev + p->rchunk_start[f];
+            const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
             const unsigned nb = blocks_for(n, kWaves);
             switch (f) {
             case F64: k_rtrn<double, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part); break;
@@ -778,9 +785,9 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
         // t = S V^T x, s = S U^T ybar (alpha = 1)
         if (D.nvpacks > 0) k_vfwd<XT><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, xx, Pb);
         for (int f = 0; f < NFMT; ++f) {
-            const int64_t n = p->rchunk_start[f + 1] - p->rchunk_start[f];
+            const int64_t n = p->rtitem_start[f + 1] - p->rtitem_start[f];
             if (n == 0) continue;
-            const int32_t* ids = p->rchunk_ids_dev + p->rchunk_start[f];
+            const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
             const unsigned nb = blocks_for(n, kWaves);
             switch (f) {
             case F64: k_rtrn<double, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb); break;
@@ -796,9 +803,9 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
             for (int f = 0; f < NFMT; ++f) {
                 if (!grad_arena[f]) continue;
                 double* gA = static_cast<double*>(grad_arena[f]);
-                const int64_t nr = p->rchunk_start[f + 1] - p->rchunk_start[f];
+                const int64_t nr = p->rtitem_start[f + 1] - p->rtitem_start[f];
                 if (nr > 0) {
-                    const int32_t* ids = p->rchunk_ids_dev + p->rchunk_start[f];
+                    const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
                     const unsigned nb = blocks_for(nr, kWaves);
                     switch (f) {
                     case F64: k_grad_r<double, XT><<<nb, kThreads, 0, st>>>(D, ids, nr, xx, yb, t, gA); break;

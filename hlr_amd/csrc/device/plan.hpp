@@ -79,6 +79,13 @@ struct VItem {          // V segment: up to kItemGroups groups of one format of 
     int64_t data_off;   // element offset of the m x (ng*W) row-major block in arena[fmt]
 };
 
+// transposed / gradient R work item: groups [g0, min(g0 + kRtSpan, ngroups)) of a chunk
+constexpr int32_t kRtSpan = 32;
+struct RtItem {
+    int32_t chunk;
+    int32_t g0;
+};
+
 struct VPack {          // forward V work unit of one wave: consecutive segments of one format,
     int32_t fmt;        // at most 64 groups in total (lane = group); everything a lane
     int32_t nseg;       // needs is stored here, so the pack starts without load chains
