@@ -2,19 +2,24 @@
 // one fixed sequence of launches.
 //
 // Every member keeps its own plan.  A batch lists the work of all members as
-// units of six kinds; one kernel per kind runs every unit of that kind of
-// every member, one unit per workgroup:
+// units of seven kinds; one kernel per kind (and per tile count for VM and
+// RM) runs every unit of that kind of every member, one unit per workgroup:
+//   XT  transposed copy of X (64 indices) for members with several
+//       right-hand sides
 //   V1  column-panel pass of members with one right-hand side (4 packs, one
 //       per wave)
-//   VM  column-panel pass on the matrix cores (one segment, nrhs >= 2)
+//   VM  column-panel pass on the matrix cores (one segment, nrhs >= 2; one
+//       kernel per count of 16-wide right-hand-side tiles)
 //   P1  first stage of the partial sums (a range of (segment, rhs) items)
 //   P2  second stage (a range of (entry, rhs) items)
 //   R1  row-panel pass, one right-hand side (one row panel)
-//   RM  row-panel pass on the matrix cores (one row panel, nrhs >= 2)
+//   RM  row-panel pass on the matrix cores (one row panel, nrhs >= 2; one
+//       kernel per tile count)
 // The launches are ordered on the caller's stream, which is the only
-// synchronisation: no workgroup waits for another.  The launch count is six
-// whatever the number of members, and the whole sequence (descriptor upload
-// plus the six kernels) can be captured in a HIP graph and replayed.
+// synchronisation: no workgroup waits for another.  The launch count is at
+// most thirteen whatever the number of members, and the whole sequence
+// (descriptor upload plus the kernels) can be captured in a HIP graph and
+// replayed.
 
 #include <hip/hip_runtime.h>
 
@@ -33,7 +38,11 @@ using namespace hmvm;
 
 namespace {
 
-enum UnitKind : int { K_V1 = 0, K_VM, K_P1, K_P2, K_R1, K_RM, kKinds };
+// matrix-core kinds come in four variants by tiles of 16 right-hand sides
+// (accumulators and LDS staging sized for ceil(R / 16) tiles of the member)
+enum UnitKind : int {
+    K_XT = 0, K_V1, K_VM1, K_VM2, K_VM3, K_VM4, K_P1, K_P2, K_R1, K_RM1, K_RM2, K_RM3, K_RM4, kKinds
+};
 constexpr int kConfigs = 4;                     // {member order, largest first} x {1024, 4096} sum items
 constexpr int kSumItems[2] = {1024, 4096};      // (segment | entry, rhs) items per P1 / P2 unit
 
@@ -46,6 +55,7 @@ struct BatchMat {
     double* part;
     double* tmp;
     double* t;
+    double* xt;               // X^T (ncols x R) for R >= 2
     int32_t R;
     int32_t pad_;
 };
@@ -74,14 +84,22 @@ __global__ __launch_bounds__(kThreads) void k_bv1(const BatchMat* __restrict__ m
 }
 
 template <typename XT>
+__global__ __launch_bounds__(kThreads) void k_bxt(const BatchMat* __restrict__ mats,
+                                                  const BatchUnit* __restrict__ units) {
+    extern __shared__ double lds[];
+    const BatchUnit w = units[blockIdx.x];
+    const BatchMat& M = mats[w.mat];
+    xtrans_tile(static_cast<const XT*>(M.x), M.ldx, M.D.ncols, M.R, M.xt, int64_t(w.first) * 64, lds, threadIdx.x);
+}
+
+template <int NTM>
 __global__ __launch_bounds__(kThreads) void k_bvm(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
-    vseg_mm_dispatch(M.D, M.D.vitems[w.first], static_cast<const XT*>(M.x), M.ldx, M.R, M.part, lds, tid, wave,
-                     lane);
+    vseg_mm_dispatch<NTM>(M.D, M.D.vitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
 }
 
 __global__ __launch_bounds__(kThreads) void k_bp1(const BatchMat* __restrict__ mats,
@@ -119,7 +137,7 @@ __global__ __launch_bounds__(kThreads) void k_br1(const BatchMat* __restrict__ m
     }
 }
 
-template <typename XT, typename YT>
+template <int NTM, typename YT>
 __global__ __launch_bounds__(kThreads) void k_brm(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     extern __shared__ double lds[];
@@ -127,8 +145,7 @@ __global__ __launch_bounds__(kThreads) void k_brm(const BatchMat* __restrict__ m
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
     const RowPanel Rw = M.D.rp[w.first];
-    rpanel_mm(M.D, Rw, M.alpha, static_cast<const XT*>(M.x), M.ldx, M.t, static_cast<YT*>(M.y), M.ldy, M.R, lds,
-              tid, wave, lane);
+    rpanel_mm<NTM>(M.D, Rw, M.alpha, M.xt, M.t, static_cast<YT*>(M.y), M.ldy, M.R, lds, tid, wave, lane);
 }
 
 inline int64_t pad32(int64_t v) { return ((v + 31) / 32) * 32; }
@@ -194,12 +211,19 @@ void launch_kinds(hmvm_batch* b, hipStream_t st) {
     const BatchUnit* u = b->d_units[c];
     const int64_t* o = b->kind_ofs[c];
     auto grid = [&](int k) { return unsigned(o[k + 1] - o[k]); };
+    if (grid(K_XT)) k_bxt<XT><<<grid(K_XT), kThreads, b->lds[K_XT], st>>>(b->d_mats, u + o[K_XT]);
     if (grid(K_V1)) k_bv1<XT><<<grid(K_V1), kThreads, b->lds[K_V1], st>>>(b->d_mats, u + o[K_V1]);
-    if (grid(K_VM)) k_bvm<XT><<<grid(K_VM), kThreads, b->lds[K_VM], st>>>(b->d_mats, u + o[K_VM]);
+    if (grid(K_VM1)) k_bvm<1><<<grid(K_VM1), kThreads, b->lds[K_VM1], st>>>(b->d_mats, u + o[K_VM1]);
+    if (grid(K_VM2)) k_bvm<2><<<grid(K_VM2), kThreads, b->lds[K_VM2], st>>>(b->d_mats, u + o[K_VM2]);
+    if (grid(K_VM3)) k_bvm<3><<<grid(K_VM3), kThreads, b->lds[K_VM3], st>>>(b->d_mats, u + o[K_VM3]);
+    if (grid(K_VM4)) k_bvm<4><<<grid(K_VM4), kThreads, b->lds[K_VM4], st>>>(b->d_mats, u + o[K_VM4]);
     if (grid(K_P1)) k_bp1<<<grid(K_P1), kThreads, 0, st>>>(b->d_mats, u + o[K_P1]);
     if (grid(K_P2)) k_bp2<<<grid(K_P2), kThreads, 0, st>>>(b->d_mats, u + o[K_P2]);
     if (grid(K_R1)) k_br1<XT, YT><<<grid(K_R1), kThreads, b->lds[K_R1], st>>>(b->d_mats, u + o[K_R1]);
-    if (grid(K_RM)) k_brm<XT, YT><<<grid(K_RM), kThreads, b->lds[K_RM], st>>>(b->d_mats, u + o[K_RM]);
+    if (grid(K_RM1)) k_brm<1, YT><<<grid(K_RM1), kThreads, b->lds[K_RM1], st>>>(b->d_mats, u + o[K_RM1]);
+    if (grid(K_RM2)) k_brm<2, YT><<<grid(K_RM2), kThreads, b->lds[K_RM2], st>>>(b->d_mats, u + o[K_RM2]);
+    if (grid(K_RM3)) k_brm<3, YT><<<grid(K_RM3), kThreads, b->lds[K_RM3], st>>>(b->d_mats, u + o[K_RM3]);
+    if (grid(K_RM4)) k_brm<4, YT><<<grid(K_RM4), kThreads, b->lds[K_RM4], st>>>(b->d_mats, u + o[K_RM4]);
 }
 
 int launch(hmvm_batch* b, hipStream_t st) {
@@ -276,7 +300,7 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
         m.D = D;
         m.R = R;
         wsoff[i] = wsz;
-        wsz += pad32(D.psize * R) + pad32(D.nps * R) + pad32(D.ntent * R) + 32;
+        wsz += pad32(D.psize * R) + pad32(D.nps * R) + pad32(D.ntent * R) + 32 + (R > 1 ? pad32(D.ncols * R) : 0);
         for (int gsel = 0; gsel < 2; ++gsel) {
             auto& K = kinds[gsel];
             if (R == 1) {
@@ -291,7 +315,10 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
                 }
             } else {
                 for (int64_t k = 0; k < D.nvitems; ++k)
-                    K[K_VM].push_back({{i, int32_t(k), 1, 0}, double(L.vitems[k].ng) * 16.0 * 64.0 * R});
+                    K[K_VM1 + (R + 15) / 16 - 1].push_back(
+                        {{i, int32_t(k), 1, 0}, double(L.vitems[k].ng) * 16.0 * 64.0 * R});
+                for (int64_t i0 = 0; i0 < D.ncols; i0 += 64)
+                    K[K_XT].push_back({{i, int32_t(i0 / 64), 1, 0}, double(std::min<int64_t>(64, D.ncols - i0)) * R});
             }
             const int items = kSumItems[gsel];
             for (int64_t k = 0; k < D.nps * R; k += items)
@@ -305,7 +332,7 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
                 double by = 0;
                 for (int64_t c = rp.chunk0; c < rp.chunk0 + rp.nchunks; ++c)
                     by += double(L.rchunks[c].ngroups) * rp.m * 16.0;
-                K[R == 1 ? K_R1 : K_RM].push_back({{i, int32_t(p), 1, 0}, by * R});
+                K[R == 1 ? K_R1 : K_RM1 + (R + 15) / 16 - 1].push_back({{i, int32_t(p), 1, 0}, by * R});
             }
         }
         b->proto.push_back(m);
@@ -323,10 +350,11 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
             b->kind_ofs[c][k + 1] = int64_t(b->units[c].size());
         }
     }
+    b->lds[K_XT] = mm_lds_bytes(maxRp);
     b->lds[K_V1] = size_t(kWaves) * kPackSegs * kXsStride * sizeof(double);
-    b->lds[K_VM] = size_t(64) * maxRp * sizeof(double);
+    for (int k = K_VM1; k <= K_VM4; ++k) b->lds[k] = mm_lds_bytes(maxRp);
     b->lds[K_R1] = size_t(kChunkGroups) * 16 * sizeof(float);
-    b->lds[K_RM] = size_t(kKB) * maxRp * sizeof(double);
+    for (int k = K_RM1; k <= K_RM4; ++k) b->lds[k] = mm_lds_bytes(maxRp);
 
     int rc;
     if ((rc = dev_alloc(b.get(), &b->d_mats, size_t(nplans))) != HMVM_OK) { release(b.get()); return rc; }
@@ -352,6 +380,7 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
         b->proto[i].part = base;
         b->proto[i].tmp = base + pad32(D.psize * R);
         b->proto[i].t = b->proto[i].tmp + pad32(D.nps * R);
+        b->proto[i].xt = (R > 1) ? b->proto[i].t + pad32(D.ntent * R) + 32 : nullptr;
     }
     *out = b.release();
     return HMVM_OK;

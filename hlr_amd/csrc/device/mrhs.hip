@@ -14,9 +14,14 @@
 //
 // V pass:  out(c, r) = sum_i V(i, c) X(x0 + i, r)   M = columns, K = rows
 // R pass:  Y(i, r)  += sum_c R(i, c) Z(c, r)        M = rows (one 16-row tile
-//          per wave), K = columns staged in LDS blocks of kKB columns.
+//          per wave), K = columns staged in LDS blocks of 64 columns.
+// Factor operands are read with one 16-byte load per lane whose W elements
+// feed W consecutive MFMAs (the K order inside a tile is free as long as A
+// and B agree).
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "mrhs.hpp"
@@ -25,14 +30,22 @@
 namespace hmvm {
 namespace {
 
-// one workgroup per V segment (x slice 64 x Rp in dynamic LDS)
+// XT = X^T (n x R row-major), 64 indices per workgroup
 template <typename XT>
-__global__ __launch_bounds__(kThr) void k_vfwd_mm(DevPlan P, const XT* __restrict__ X, int64_t ldx, int R,
+__global__ __launch_bounds__(kThr) void k_xtrans(const XT* __restrict__ X, int64_t ldx, int64_t n, int R,
+                                                 double* __restrict__ XTb) {
+    extern __shared__ double tile[];
+    xtrans_tile(X, ldx, n, R, XTb, int64_t(blockIdx.x) * 64, tile, threadIdx.x);
+}
+
+// one workgroup per V segment (x slice 64 x Rp in dynamic LDS)
+template <int NTM>
+__global__ __launch_bounds__(kThr) void k_vfwd_mm(DevPlan P, const double* __restrict__ XTb, int R,
                                                   double* __restrict__ Pb) {
     extern __shared__ double xs[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const VItem sg = P.vitems[blockIdx.x];
-    vseg_mm_dispatch(P, sg, X, ldx, R, Pb, xs, tid, wave, lane);
+    vseg_mm_dispatch<NTM>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
 }
 
 __global__ __launch_bounds__(256) void k_psum1_mm(DevPlan D, const double* __restrict__ Pb, double* __restrict__ tmp,
@@ -47,14 +60,14 @@ __global__ __launch_bounds__(256) void k_psum2_mm(DevPlan D, const double* __res
     if (g < D.ntent * R) psum2_item(D, tmp, t, R, alpha, g);
 }
 
-template <typename XT, typename YT>
-__global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const XT* __restrict__ X, int64_t ldx,
+template <int NTM, typename YT>
+__global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const double* __restrict__ XTb,
                                                   const double* __restrict__ t, YT* __restrict__ Y, int64_t ldy,
                                                   int R) {
-    extern __shared__ double zbuf[];               // kKB x Rp, sized at launch
+    extern __shared__ double zbuf[];               // 64 x kMmStride, sized at launch
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const RowPanel Rw = P.rp[blockIdx.x];
-    rpanel_mm(P, Rw, alpha, X, ldx, t, Y, ldy, R, zbuf, tid, wave, lane);
+    rpanel_mm<NTM>(P, Rw, alpha, XTb, t, Y, ldy, R, zbuf, tid, wave, lane);
 }
 
 } // namespace
@@ -64,38 +77,53 @@ int launch_apply_mrhs(const DevPlan& D, int64_t max_ws_part, double alpha, const
     if (R < 2 || R > kMaxRp) return 1;
     (void)max_ws_part;
     const int64_t pad = 32;
+    // workspace: partials | stage-1 sums | t | XT (ncols x R)
     double* part = ws;
     double* tmp = part + ((D.psize * R + pad - 1) / pad) * pad;
     double* t = tmp + ((D.nps * R + pad - 1) / pad) * pad;
+    double* xt = t + ((D.ntent * R + pad - 1) / pad) * pad + pad;
     auto blocks = [](int64_t n, int per) { return unsigned((n + per - 1) / per); };
     const int Rp = (R + 15) & ~15;
-    const size_t lds_v = size_t(64) * Rp * sizeof(double);
-    const size_t lds_r = size_t(kKB) * Rp * sizeof(double);
-    if (D.nvitems > 0) {
+    const size_t lds = mm_lds_bytes(Rp);
+    if (D.ncols > 0) {
         if (vdt == 0)
-            k_vfwd_mm<double><<<unsigned(D.nvitems), kThr, lds_v, st>>>(D, static_cast<const double*>(x), ldx, R, part);
+            k_xtrans<double><<<blocks(D.ncols, 64), kThr, lds, st>>>(static_cast<const double*>(x), ldx, D.ncols, R, xt);
         else
-            k_vfwd_mm<float><<<unsigned(D.nvitems), kThr, lds_v, st>>>(D, static_cast<const float*>(x), ldx, R, part);
+            k_xtrans<float><<<blocks(D.ncols, 64), kThr, lds, st>>>(static_cast<const float*>(x), ldx, D.ncols, R, xt);
     }
+    // accumulators sized for exactly the ceil(R / 16) tiles of this call
+    auto vpass = [&](auto ntm) {
+        constexpr int NTM = decltype(ntm)::value;
+        k_vfwd_mm<NTM><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
+    };
+    auto rpass = [&](auto ntm) {
+        constexpr int NTM = decltype(ntm)::value;
+        if (vdt == 0)
+            k_ufwd_mm<NTM, double><<<unsigned(D.nrp), kThr, lds, st>>>(D, alpha, xt, t, static_cast<double*>(y), ldy, R);
+        else
+            k_ufwd_mm<NTM, float><<<unsigned(D.nrp), kThr, lds, st>>>(D, alpha, xt, t, static_cast<float*>(y), ldy, R);
+    };
+    auto by_tiles = [&](auto pass) {
+        switch (Rp / 16) {
+        case 1: pass(std::integral_constant<int, 1>()); break;
+        case 2: pass(std::integral_constant<int, 2>()); break;
+        case 3: pass(std::integral_constant<int, 3>()); break;
+        default: pass(std::integral_constant<int, 4>()); break;
+        }
+    };
+    if (D.nvitems > 0) by_tiles(vpass);
     if (D.ntent > 0) {
         if (D.nps > 0) k_psum1_mm<<<blocks(D.nps * R, 256), 256, 0, st>>>(D, part, tmp, R);
         k_psum2_mm<<<blocks(D.ntent * R, 256), 256, 0, st>>>(D, tmp, t, R, alpha);
     }
-    if (D.nrp > 0) {
-        if (vdt == 0)
-            k_ufwd_mm<double, double><<<unsigned(D.nrp), kThr, lds_r, st>>>(D, alpha, static_cast<const double*>(x), ldx,
-                                                                           t, static_cast<double*>(y), ldy, R);
-        else
-            k_ufwd_mm<float, float><<<unsigned(D.nrp), kThr, lds_r, st>>>(D, alpha, static_cast<const float*>(x), ldx, t,
-                                                                         static_cast<float*>(y), ldy, R);
-    }
+    if (D.nrp > 0) by_tiles(rpass);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
 int64_t mrhs_ws_doubles(const DevPlan& D, int R) {
     const int64_t pad = 32;
     return ((D.psize * R + pad - 1) / pad) * pad + ((D.nps * R + pad - 1) / pad) * pad +
-           ((D.ntent * R + pad - 1) / pad) * pad + pad;
+           ((D.ntent * R + pad - 1) / pad) * pad + pad + ((D.ncols * R + pad - 1) / pad) * pad;
 }
 
 } // namespace hmvm

@@ -15,7 +15,6 @@
 namespace hmvm {
 
 constexpr int kThr = 256;
-constexpr int kKB = 32;              // columns per LDS block of the R pass
 constexpr int kMaxRp = 64;           // right-hand sides per pass (tiles of 16)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -25,7 +24,6 @@ template <typename T> struct Mm;
 template <> struct Mm<double> {
     using A = double; using V4 = d4;
     static constexpr int F = F64; static constexpr int W = 2;
-    __device__ static inline double get(const double* p, int64_t i) { return p[i]; }
     __device__ static inline V4 mma(double a, double b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     // tile row of element e held by lanes with lane / 16 == kk
     __device__ static inline int drow(int kk, int e) { return kk + 4 * e; }
@@ -33,92 +31,161 @@ template <> struct Mm<double> {
 template <> struct Mm<float> {
     using A = float; using V4 = f4;
     static constexpr int F = F32; static constexpr int W = 4;
-    __device__ static inline float get(const float* p, int64_t i) { return p[i]; }
     __device__ static inline V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static inline int drow(int kk, int e) { return 4 * kk + e; }
 };
 template <> struct Mm<uint16_t> {
     using A = float; using V4 = f4;
     static constexpr int F = BF16; static constexpr int W = 8;
-    __device__ static inline float get(const uint16_t* p, int64_t i) { return __uint_as_float(unsigned(p[i]) << 16); }
     __device__ static inline V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static inline int drow(int kk, int e) { return 4 * kk + e; }
 };
 template <> struct Mm<uint8_t> {
     using A = float; using V4 = f4;
     static constexpr int F = FP8; static constexpr int W = 16;
-    __device__ static inline float get(const uint8_t* p, int64_t i) {
-        return __builtin_amdgcn_cvt_f32_fp8(int(p[i]), 0);
-    }
     __device__ static inline V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static inline int drow(int kk, int e) { return 4 * kk + e; }
 };
+
+// MFMA steps whose factor loads are issued together: 8 for FP64 / FP32,
+// fewer for the narrow formats, whose loads each feed more MFMAs
+template <int W> struct MmSteps { static constexpr int value = (W <= 4) ? 8 : (W == 8 ? 4 : 2); };
 
 template <typename T> struct IsF8 { static constexpr bool v = false; };
 template <> struct IsF8<uint8_t> { static constexpr bool v = true; };
 
 // ---------------------------------------------------------------------------
-// V pass: one workgroup per segment, out(c, r) = sum_i V(i, c) X(x0 + i, r);
-// the 4 waves split the 16-column tiles, x slice (64 x Rp) shared in LDS
+// V pass: one workgroup per segment, out(c, r) = sum_i V(i, c) X(x0 + i, r),
+// computed as out^T = X^T V: M = right-hand sides, N = columns, K = rows.
+// Wave w takes groups 16 w .. 16 w + 15 of the segment; lane n + 16 k loads
+// row 4 s + k of group 16 w + n with one 16-byte load, whose element j is
+// B[k][n] of MFMA j (column W (16 w + n) + j).  A[m][k] = X(row 4 s + k,
+// rhs m) comes from the x slice in LDS (64 rows x kMmStride).
+// Accumulators: JB elements x up to 4 rhs tiles; formats with W > JB make
+// W / JB passes over the segment.
 // ---------------------------------------------------------------------------
 
-template <typename T, typename XT>
-__device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const XT* __restrict__ X, int64_t ldx, int R,
+// ---------------------------------------------------------------------------
+// XT = X^T, row-major n x R: the right-hand sides of one index are
+// contiguous, so the passes below stage x and z values with the rhs index
+// fastest and every read is a contiguous run of R values.  One workgroup
+// transposes 64 indices through LDS (reads of X coalesced along the index,
+// writes of XT coalesced along the rhs).
+// ---------------------------------------------------------------------------
+
+template <typename XT>
+__device__ inline void xtrans_tile(const XT* __restrict__ X, int64_t ldx, int64_t n, int R, double* __restrict__ XTb,
+                                   int64_t i0, double* tile, int tid) {
+    constexpr int TS = kMaxRp + 1;                       // [64][TS]: index, rhs
+    for (int idx = tid; idx < 64 * R; idx += kThr) {
+        const int i = idx & 63, r = idx >> 6;
+        tile[i * TS + r] = (i0 + i < n) ? double(X[int64_t(r) * ldx + i0 + i]) : 0.0;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 64 * R; idx += kThr) {
+        const int i = idx / R, r = idx - i * R;
+        if (i0 + i < n) XTb[(i0 + i) * R + r] = tile[i * TS + r];
+    }
+}
+
+// LDS of one matrix-core workgroup: 64 rows (V pass) or 64 columns (R pass)
+// of up to kMaxRp right-hand sides, with the compile-time row stride kMmStride
+// (odd, so that neighbouring rows start in different banks; being constant,
+// every operand read is one base register plus an immediate offset)
+constexpr int kMmStride = kMaxRp + 1;
+__host__ __device__ inline size_t mm_lds_bytes(int) { return size_t(64) * kMmStride * sizeof(double); }
+
+template <int NTM, typename T>
+__device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const double* __restrict__ XTb, int R,
                                   double* __restrict__ Pb, double* xsraw, int tid, int wave, int lane) {
     using M = Mm<T>;
     using A = typename M::A;
+    using E = Elem<T>;
     constexpr int W = M::W;
+    constexpr int JB = (W < 4) ? W : 4;
+    constexpr int NB = MmSteps<W>::value;
     const ColPanel C = P.cp[sg.q];
     const int m = C.m;
     const int Rp = (R + 15) & ~15;
-    A* xs = reinterpret_cast<A*>(xsraw);                 // [64][Rp]
-    for (int idx = tid; idx < 64 * Rp; idx += kThr) {
-        const int i = idx / Rp, r = idx % Rp;
-        xs[idx] = (i < m && r < R) ? A(X[int64_t(r) * ldx + C.col0 + i]) : A(0);
+    const int NT = Rp / 16;
+    constexpr int XS = kMmStride;
+    A* xs = reinterpret_cast<A*>(xsraw);                 // [64][XS]: row, rhs
+    constexpr int RS = 16 * NTM;                         // staged rhs slots (constant divisor)
+    for (int idx = tid; idx < 64 * RS; idx += kThr) {
+        const int i = idx / RS, r = idx - i * RS;         // rhs fastest: contiguous reads of XT
+        if (r < Rp) xs[i * XS + r] = (i < m && r < R) ? A(XTb[(C.col0 + i) * R + r]) : A(0);
     }
     __syncthreads();
-    const int ncol = sg.ng * W;
-    const int64_t ld = ncol;
-    const T* base = static_cast<const T*>(P.arena[M::F]) + sg.data_off;
+    if (16 * wave >= sg.ng) return;                      // no barrier follows
+    const int n = lane & 15, kk = lane >> 4;
+    const int gl = 16 * wave + n;
+    const bool gon = gl < sg.ng;
+    const int64_t ld = int64_t(sg.ng) * W;
+    const T* base = static_cast<const T*>(P.arena[M::F]) + sg.data_off + int64_t(gon ? gl : 0) * W;
     const int64_t vc0 = C.vcol0[M::F] + int64_t(sg.g0) * W;
-    const int ci = lane & 15, kk = lane >> 4;
-    for (int c0 = 16 * wave; c0 < ncol; c0 += 16 * 4) {
-        A a[16];
+    for (int jp = 0; jp < W; jp += JB) {
+        typename M::V4 acc[JB][NTM];
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int i = 4 * s + kk, c = c0 + ci;
-            a[s] = (i < m && c < ncol) ? A(M::get(base, int64_t(i) * ld + c)) : A(0);
-        }
-        for (int r0 = 0; r0 < Rp; r0 += 16) {
-            typename M::V4 acc = {0, 0, 0, 0};
+        for (int j = 0; j < JB; ++j)
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const A b = xs[(4 * s + kk) * Rp + r0 + ci];
-                acc = M::mma(a[s], b, acc);
+            for (int nt = 0; nt < NTM; ++nt) acc[j][nt] = typename M::V4{0, 0, 0, 0};
+        // NB steps of 4 rows: their loads are all in flight before the MFMAs
+        for (int s0 = 0; 4 * s0 < m; s0 += NB) {
+            u32x4 raw[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int row = 4 * (s0 + b) + kk;
+                raw[b] = (gon && row < m) ? raw16(base + int64_t(row) * ld) : u32x4{0u, 0u, 0u, 0u};
             }
-            const int r = r0 + ci;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int c = c0 + M::drow(kk, e);
-                if (c < ncol && r < R) {
-                    const int32_t d = P.vdest[vc0 + c];
-                    double v = double(acc[e]);
-                    if (IsF8<T>::v) v *= double(P.vscale[vc0 + c]);
-                    if (d >= 0) Pb[int64_t(d) * R + r] = v;
+            for (int b = 0; b < NB; ++b) {
+                if (4 * (s0 + b) >= m) break;            // uniform
+                const int row = 4 * (s0 + b) + kk;
+                A v[W];
+                E::decode(raw[b], v);
+#pragma unroll
+                for (int nt = 0; nt < NTM; ++nt) {
+                    if (nt < NT) {
+                        const A a = xs[row * XS + nt * 16 + n];
+#pragma unroll
+                        for (int j = 0; j < JB; ++j) acc[j][nt] = M::mma(a, v[jp + j], acc[j][nt]);
+                    }
+                }
+            }
+        }
+        // D[mr][n] of MFMA j: rhs nt * 16 + mr, column W gl + jp + j
+        if (gon) {
+#pragma unroll
+            for (int j = 0; j < JB; ++j) {
+                const int64_t vc = vc0 + int64_t(gl) * W + jp + j;
+                const int32_t d = P.vdest[vc];
+                if (d < 0) continue;
+                const double sc = IsF8<T>::v ? double(P.vscale[vc]) : 1.0;
+#pragma unroll
+                for (int nt = 0; nt < NTM; ++nt) {
+                    if (nt < NT) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int r = nt * 16 + M::drow(kk, e);
+                            if (r < R) Pb[int64_t(d) * R + r] = double(acc[j][nt][e]) * sc;
+                        }
+                    }
                 }
             }
         }
     }
 }
 
-template <typename XT>
-__device__ inline void vseg_mm_dispatch(const DevPlan& P, const VItem& sg, const XT* __restrict__ X, int64_t ldx,
+// NTM: tiles of 16 right-hand sides the accumulators are sized for
+// (compile time); the call handles ceil(R / 16) <= NTM of them
+template <int NTM>
+__device__ inline void vseg_mm_dispatch(const DevPlan& P, const VItem& sg, const double* __restrict__ XTb,
                                         int R, double* __restrict__ Pb, double* xs, int tid, int wave, int lane) {
     switch (sg.fmt) {
-    case F64: vseg_mm_wg<double>(P, sg, X, ldx, R, Pb, xs, tid, wave, lane); break;
-    case F32: vseg_mm_wg<float>(P, sg, X, ldx, R, Pb, xs, tid, wave, lane); break;
-    case BF16: vseg_mm_wg<uint16_t>(P, sg, X, ldx, R, Pb, xs, tid, wave, lane); break;
-    default: vseg_mm_wg<uint8_t>(P, sg, X, ldx, R, Pb, xs, tid, wave, lane); break;
+    case F64: vseg_mm_wg<NTM, double>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
+    case F32: vseg_mm_wg<NTM, float>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
+    case BF16: vseg_mm_wg<NTM, uint16_t>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
+    default: vseg_mm_wg<NTM, uint8_t>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
     }
 }
 
@@ -152,72 +219,97 @@ __device__ inline void psum2_item(const DevPlan& D, const double* __restrict__ t
 // R pass: one workgroup per row panel, wave w owns rows 16w .. 16w + 15
 // ---------------------------------------------------------------------------
 
-template <typename T, typename XT>
-__device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alpha, const XT* __restrict__ X,
-                          int64_t ldx, const double* __restrict__ t, int R, int Rp, void* zraw,
-                          typename Mm<T>::V4 (&acc)[kMaxRp / 16], int tid, int wave, int lane) {
+// R pass of one chunk: Y(i, r) += sum_c R(i, c) Z(c, r) with M = rows (16
+// per wave), N = right-hand sides, K = columns.  Z is staged 64 columns at
+// a time in LDS (64 x kMmStride); lane i + 16 k loads row 16 w + i of group
+// g0 + k with one 16-byte load, whose element j is A[i][k] of MFMA j
+// (column W (g0 + k) + j), and B[k][n] = Z(W (g0 + k) + j, rhs n).
+template <int NTM, typename T>
+__device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alpha, const double* __restrict__ XTb,
+                          const double* __restrict__ t, int R, int Rp, void* zraw,
+                          typename Mm<T>::V4 (&acc)[NTM], int tid, int wave, int lane) {
     using M = Mm<T>;
     using A = typename M::A;
+    using E = Elem<T>;
     constexpr int W = M::W;
-    A* zs = static_cast<A*>(zraw);                       // [kKB][Rp]
+    constexpr int GB = 64 / W;                           // groups per staged block of 64 columns
+    constexpr int NB = MmSteps<W>::value;
+    constexpr int RS = 16 * NTM;                         // staged rhs slots (constant divisor)
+    A* zs = static_cast<A*>(zraw);                       // [64][ZS]: column, rhs
+    constexpr int ZS = kMmStride;
+    const int NT = Rp / 16;
     const int ncol = Ch.ngroups * W;
-    const T* base = static_cast<const T*>(P.arena[M::F]) + Ch.data_off;
     const int row = 16 * wave + (lane & 15);
-    const int kk = lane >> 4;
-    for (int k0 = 0; k0 < ncol; k0 += kKB) {
+    const int kk = lane >> 4, n = lane & 15;
+    const T* base = static_cast<const T*>(P.arena[M::F]) + Ch.data_off + int64_t(row < m ? row : 0) * W;
+    for (int gb0 = 0; gb0 < Ch.ngroups; gb0 += GB) {
         __syncthreads();
-        for (int idx = tid; idx < kKB * Rp; idx += kThr) {
-            const int c = k0 + idx / Rp, r = idx % Rp;
+        for (int idx = tid; idx < 64 * RS; idx += kThr) {
+            const int c = idx / RS, r = idx - c * RS;    // rhs fastest: contiguous reads of XT and t
+            if (r >= Rp) continue;
+            const int col = gb0 * W + c;
             double z = 0;
-            if (c < ncol && r < R) {
-                const int32_t s = P.colsrc[Ch.col0 + c];
-                z = (s >= 0) ? alpha * double(X[int64_t(r) * ldx + s]) : (s == kPadCol ? 0.0 : t[int64_t(~s) * R + r]);
-                if (IsF8<T>::v) z *= double(P.zscale[Ch.col0 + c]);
+            if (col < ncol && r < R) {
+                const int32_t s = P.colsrc[Ch.col0 + col];
+                z = (s >= 0) ? alpha * XTb[int64_t(s) * R + r] : (s == kPadCol ? 0.0 : t[int64_t(~s) * R + r]);
+                if (IsF8<T>::v) z *= double(P.zscale[Ch.col0 + col]);
             }
-            zs[idx] = A(z);
+            zs[c * ZS + r] = A(z);
         }
         __syncthreads();
-        // A operands of the whole block first (independent loads), then the MFMAs
-        A a[kKB / 4];
+        const int gend = min(gb0 + GB, int(Ch.ngroups));
+        // NB steps of 4 groups: their loads are all in flight before the MFMAs
+        for (int g0 = gb0; g0 < gend; g0 += 4 * NB) {
+            u32x4 raw[NB];
 #pragma unroll
-        for (int s = 0; s < kKB / 4; ++s) {
-            const int c = k0 + 4 * s + kk;
-            a[s] = (row < m && c < ncol) ? A(M::get(base, int64_t(c / W) * m * W + int64_t(row) * W + (c % W)))
-                                         : A(0);
-        }
+            for (int b = 0; b < NB; ++b) {
+                const int g = g0 + 4 * b + kk;
+                raw[b] = (row < m && g < gend) ? raw16(base + int64_t(g) * m * W) : u32x4{0u, 0u, 0u, 0u};
+            }
 #pragma unroll
-        for (int s = 0; s < kKB / 4; ++s) {
-            for (int nt = 0; nt < Rp / 16; ++nt) {
-                const A b = zs[(4 * s + kk) * Rp + nt * 16 + (lane & 15)];
-                acc[nt] = M::mma(a[s], b, acc[nt]);
+            for (int b = 0; b < NB; ++b) {
+                if (g0 + 4 * b >= gend) break;           // uniform
+                const int g = g0 + 4 * b + kk;
+                A v[W];
+                E::decode(raw[b], v);
+                const A* zg = zs + ((g - gb0) * W) * ZS + n;
+#pragma unroll
+                for (int nt = 0; nt < NTM; ++nt) {
+                    if (nt < NT) {
+#pragma unroll
+                        for (int j = 0; j < W; ++j) acc[nt] = M::mma(v[j], zg[j * ZS + nt * 16], acc[nt]);
+                    }
+                }
             }
         }
     }
 }
 
 // whole row panel, Y(:, 0..R) += R_p Z; all kThr threads call it
-template <typename XT, typename YT>
-__device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double alpha, const XT* __restrict__ X,
-                                 int64_t ldx, const double* __restrict__ t, YT* __restrict__ Y, int64_t ldy, int R,
+template <int NTM, typename YT>
+__device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double alpha, const double* __restrict__ XTb,
+                                 const double* __restrict__ t, YT* __restrict__ Y, int64_t ldy, int R,
                                  double* zbuf, int tid, int wave, int lane) {
     const int Rp = (R + 15) & ~15;
-    d4 acc64[kMaxRp / 16];
-    f4 acc32[kMaxRp / 16];
+    d4 acc64[NTM];
+    f4 acc32[NTM];
 #pragma unroll
-    for (int i = 0; i < kMaxRp / 16; ++i) { acc64[i] = d4{0, 0, 0, 0}; acc32[i] = f4{0, 0, 0, 0}; }
+    for (int i = 0; i < NTM; ++i) { acc64[i] = d4{0, 0, 0, 0}; acc32[i] = f4{0, 0, 0, 0}; }
     for (int ci = 0; ci < Rw.nchunks; ++ci) {
         const RChunk Ch = P.rchunks[Rw.chunk0 + ci];
         switch (Ch.fmt) {
-        case F64: rchunk_mm<double>(P, Ch, Rw.m, alpha, X, ldx, t, R, Rp, zbuf, acc64, tid, wave, lane); break;
-        case F32: rchunk_mm<float>(P, Ch, Rw.m, alpha, X, ldx, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
-        case BF16: rchunk_mm<uint16_t>(P, Ch, Rw.m, alpha, X, ldx, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
-        default: rchunk_mm<uint8_t>(P, Ch, Rw.m, alpha, X, ldx, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
+        case F64: rchunk_mm<NTM, double>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc64, tid, wave, lane); break;
+        case F32: rchunk_mm<NTM, float>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
+        case BF16: rchunk_mm<NTM, uint16_t>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
+        default: rchunk_mm<NTM, uint8_t>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
         }
     }
     // bring the FP64 tiles into the FP32 element order (row 4 kk + e), then
     // one writer per output element: row 16 wave + 4 (lane / 16) + e
     const int kk = lane >> 4;
-    for (int nt = 0; nt < Rp / 16; ++nt) {
+#pragma unroll
+    for (int nt = 0; nt < NTM; ++nt) {
+        if (nt >= Rp / 16) break;
         double v64[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {

@@ -241,10 +241,12 @@ int hmvm_batch_destroy(hmvm_batch* batch);
 /*
  * y_i += alpha_i A_i x_i for every plan i (forward products; nrhs[i] in
  * 1..64 right-hand sides, column-major with ldx[i] / ldy[i]).  The work of
- * all members runs as six stream-ordered kernels (column-panel pass for one
- * and for several right-hand sides, two partial-sum stages, row-panel pass
- * for one and for several right-hand sides), one unit of work per
- * workgroup, whatever the number of members.  No counterpart in the
+ * all members runs as a fixed sequence of at most thirteen stream-ordered
+ * kernels (transposed copy of X, column-panel pass for one and for several
+ * right-hand sides, two partial-sum stages, row-panel pass for one and for
+ * several right-hand sides; the matrix-core passes once per count of
+ * 16-wide right-hand-side tiles), one unit of work per workgroup, whatever
+ * the number of members.  No counterpart in the
  * reference (it applies one matrix at a time, programs/mvm/mvm.hh:151-204).
  */
 int hmvm_batch_apply(hmvm_batch* batch, const double* alphas,
