@@ -43,8 +43,8 @@ namespace {
 enum UnitKind : int {
     K_XT = 0, K_V1, K_VM1, K_VM2, K_VM3, K_VM4, K_P1, K_P2, K_R1, K_RM1, K_RM2, K_RM3, K_RM4, kKinds
 };
-constexpr int kConfigs = 4;                     // {member order, largest first} x {1024, 4096} sum items
-constexpr int kSumItems[2] = {1024, 4096};      // (segment | entry, rhs) items per P1 / P2 unit
+constexpr int kConfigs = 4;                     // {member order, largest first} x {256, 1024} sum items
+constexpr int kSumItems[2] = {256, 1024};       // (segment | entry, rhs) items per P1 / P2 unit
 
 struct BatchMat {
     DevPlan D;

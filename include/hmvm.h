@@ -261,7 +261,7 @@ int hmvm_batch_replay(hmvm_batch* batch, void* stream);
 
 /*
  * In-project autotuner: times the candidate schedules (unit order {member
- * order, largest unit first} x partial-sum items per workgroup {1024, 4096})
+ * order, largest unit first} x partial-sum items per workgroup {256, 1024})
  * on `stream` with the given vectors (y accumulates alpha = 1 products, the
  * caller resets it) and keeps the fastest; *chosen receives
  * 2 * order + index of the item count.
