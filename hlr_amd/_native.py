@@ -109,6 +109,7 @@ HMVM_SYMBOLS = [
     ("hmvm_batch_check", _i32, [_vp, _vp]),
     ("hmvm_batch_tune", _i32, [_vp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_vp), C.POINTER(_i64), _vp,
                                C.POINTER(_i32)]),
+    ("hmvm_spectral_norm", _i32, [_vp, C.c_double, _i64, _i32, _vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
     ("hmvm_error_string", C.c_char_p, [_i32]),
 ]
 

@@ -6,6 +6,7 @@
 //   hlr::hip::tlr::mul_vec(alpha, op, M, x, y) (tbb/arith.hh:998-1042)
 //   hlr::hip::hip_arithmetic                   (tbb/arith.hh:1414-1510,
 //                                               arith/defaults.hh:20-104)
+//   hlr::hip::norm::spectral(arith, A, ...)    (arith/norm.hh:89-101)
 // with y += alpha op(M) x, alpha == 0 a no-op, op in {apply_normal,
 // apply_conjugate, apply_transposed, apply_adjoint}, and errors thrown as
 // std::runtime_error (HLR_ASSERT/HLR_ERROR, utils/log.hh:40-67).  The matrix
@@ -152,6 +153,21 @@ struct hip_arithmetic {
         hlr::hip::mul_vec(alpha, op, M, x, y, stream);
     }
 };
+
+namespace norm {
+
+// spectral norm by power iteration on the device (arith/detail/norm.hh:448-527,
+// called as hlr::norm::spectral(arithmetic, A, atol, amax_it, squared),
+// arith/norm.hh:89-101)
+inline double spectral(const hip_arithmetic& arith, const hmatrix& A, const double atol = 0,
+                       const size_t amax_it = 0, const bool squared = true) {
+    double lambda = 0;
+    check(hmvm_spectral_norm(A.plan(), atol, int64_t(amax_it), squared ? 1 : 0, arith.stream, &lambda, nullptr),
+          "hmvm_spectral_norm");
+    return lambda;
+}
+
+} // namespace norm
 
 } // namespace hip
 } // namespace hlr

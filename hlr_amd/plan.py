@@ -136,6 +136,18 @@ class Plan:
                                      C.c_void_p(y.data_ptr()), _vec_dtype(x), _stream_handle(stream))
         N.check(rc, "hmvm_tlr_apply")
 
+    def spectral_norm(self, atol: float = 0.0, max_it: int = 0, squared: bool = True, stream=None):
+        """Spectral norm by power iteration on the device (the reference's
+        hlr::norm::spectral, arith/detail/norm.hh:448-527): returns
+        (lambda, iterations).  atol = 0: sqrt(eps); max_it = 0:
+        max(5, min(nrows, ncols) / 10)."""
+        lam = C.c_double()
+        its = C.c_int64()
+        N.check(N.hmvm().hmvm_spectral_norm(self._p, float(atol), int(max_it), 1 if squared else 0,
+                                            _stream_handle(stream), C.byref(lam), C.byref(its)),
+                "hmvm_spectral_norm")
+        return float(lam.value), int(its.value)
+
     # -- arenas and gradients -------------------------------------------------
     def arena_elems(self, fmt: int) -> int:
         e = C.c_int64()

@@ -273,6 +273,25 @@ int hmvm_batch_tune(hmvm_batch* batch, const void* const* xs,
 /* synchronise `stream` and report a device error of the batch's launches */
 int hmvm_batch_check(hmvm_batch* batch, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* consumers of the product                                                 */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Spectral norm of the plan's matrix by power iteration, replacing
+ * hlr::norm::spectral(arithmetic, A, atol, amax_it, squared)
+ * (arith/norm.hh:89-101, arith/detail/norm.hh:448-527): same start vector
+ * (std::default_random_engine, uniform in [-1, 1], normalised), same
+ * iteration (squared: y = A^H A x, lambda = sqrt(|<x, y>|); otherwise
+ * y = A x, lambda = |y|, square matrices only), same stopping rules and
+ * defaults (atol == 0: sqrt(eps); amax_it == 0: max(5, min(nrows, ncols) /
+ * 10)).  Every product is an hmvm_apply on `stream`; synchronises `stream`.
+ * *iterations (may be NULL) receives the number of steps taken.
+ */
+int hmvm_spectral_norm(const hmvm_plan* plan, double atol, int64_t amax_it,
+                       int squared, void* stream, double* norm,
+                       int64_t* iterations);
+
 const char* hmvm_error_string(int code);
 
 #ifdef __cplusplus
