@@ -10,6 +10,10 @@ One step = one H-MVM y += 2 A x over the whole matrix.  The mixed-precision
 plan (per-column FP32/BF16/FP8-E4M3 chosen from singular values against eps,
 dense leaves FP32, FP32 accumulation inside panels) is the measured path;
 the plain FP64 plan is timed beside it, both with factors resident in HBM.
+A third plan stores every column in block floating point (values of 8 to 25
+bits with a power-of-two step per panel column, width chosen per column so
+that the column's 2-norm error stays within the same VALR budget); its
+figures are reported beside the mixed-precision ones under "bfp".
 The relative error of the mixed-precision result is taken against the FP64
 GPU result (which the full-size parity tests hold to <= 1e-13 of the FP64
 oracle on this matrix).  The CPU baseline is the oracle's restatement of the
@@ -50,8 +54,9 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=1, help="right-hand sides per call")
     ap.add_argument("--op", choices=["normal", "transposed"], default="normal",
                     help="product timed (the square sphere matrix keeps the same sizes)")
-    ap.add_argument("--precision", choices=["both", "fp64", "valr"], default="both",
-                    help="time only one plan (profiling runs); the JSON line needs both")
+    ap.add_argument("--precision", choices=["both", "fp64", "valr", "bfp"], default="both",
+                    help="time only one plan (profiling runs); the JSON line needs fp64 and valr")
+    ap.add_argument("--no-bfp", action="store_true", help="skip the block floating point plan")
     ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
     ap.add_argument("--workload", choices=["sphere", "batch"], default="sphere",
                     help="sphere: 1/r, n = 262144 (default); batch: 64 Matern matrices, many right-hand sides")
@@ -163,6 +168,21 @@ def main() -> int:
         log(f"valr: {msmp:.3f} ms/MVM  phases {phmp.round(4).tolist()} ms")
         pmp.close()
 
+    # ---- block floating point plan ----
+    bfp = None
+    if args.precision == "bfp" or (args.precision == "both" and not args.no_bfp):
+        t0 = time.time()
+        pbf = Plan.from_hmatrix(hm, precision="bfp", eps=args.eps)
+        ibf = pbf.info()
+        log(f"bfp plan: {ibf['factor_bytes'] / 1e9:.2f} GB cols/format {ibf['cols_by_format']} "
+            f"in {time.time() - t0:.1f} s")
+        msbf, phbf = time_plan(pbf, args.steps, args.warmup)
+        ybf = result(pbf)
+        log(f"bfp: {msbf:.3f} ms/MVM  phases {phbf.round(4).tolist()} ms")
+        pbf.close()
+        torch.cuda.empty_cache()
+        bfp = {"ms_per_step": msbf, "info": ibf, "phases": phbf, "y": ybf}
+
     if args.precision != "both":
         log("single-precision profiling run: no JSON line")
         return 0
@@ -250,6 +270,19 @@ def main() -> int:
                      "algorithmic_bytes": r_bytes},
         "cpu_baseline": cpu,
     }
+    if bfp is not None:
+        db = (bfp["y"] - y64).reshape(R, -1)
+        err_bf = float(max(np.linalg.norm(db[j]) / np.linalg.norm(y64.reshape(R, -1)[j]) for j in range(R)))
+        ib = bfp["info"]
+        bb = ib["datasize"] + 8.0 * 2 * (R - 1) * (n + n)
+        bf = {"mvm_per_s": 1000.0 * R / bfp["ms_per_step"], "ms_per_step": bfp["ms_per_step"],
+              "speedup_vs_fp64": ms64 / bfp["ms_per_step"], "rel_err_vs_fp64_gpu": err_bf,
+              "bytes_per_mvm": bb, "hbm_gbs": bb / (bfp["ms_per_step"] * 1e-3) / 1e9,
+              "phases_ms": bfp["phases"].tolist(), "cols_by_format": ib["cols_by_format"]}
+        if R == 1 and OP == N.OP_NORMAL and bfp["phases"][2] > 0:
+            rb = ib["bytes_rpass"] + 8.0 * (n + n)
+            bf["rpass_gbs"] = rb / (bfp["phases"][2] * 1e-3) / 1e9
+        out["bfp"] = bf
     print(json.dumps(out), flush=True)
     return 0
 

@@ -21,9 +21,13 @@ ERRORS = {
 }
 OP_NORMAL, OP_TRANSPOSED, OP_ADJOINT, OP_CONJUGATE = 0, 1, 2, 3
 BLOCK_DENSE, BLOCK_LOWRANK, BLOCK_LOWRANK_SV = 0, 1, 2
-PREC_FP64, PREC_FP32, PREC_VALR = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_VALR, PREC_BFP = 0, 1, 2, 3
 F64, F32 = 0, 1
 FMT_F64, FMT_F32, FMT_BF16, FMT_FP8 = 0, 1, 2, 3
+FMT_B8, FMT_B10, FMT_B12, FMT_B16, FMT_B21, FMT_B25 = 4, 5, 6, 7, 8, 9
+NFMT = 10
+# bits per stored value of each format; a 16-byte row holds 128 // bits values
+FMT_BITS = (64, 32, 16, 8, 8, 10, 12, 16, 21, 25)
 
 KERNEL_LOG, KERNEL_LOG_GALERKIN, KERNEL_MATERN, KERNEL_INVR = 0, 1, 2, 3
 GEOM_LINE, GEOM_GRID1D, GEOM_CUBE, GEOM_SPHERE = 0, 1, 2, 3
@@ -46,7 +50,7 @@ class HmvmPlanInfo(C.Structure):
     _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("nblocks", C.c_int64),
                 ("ndense", C.c_int64), ("nlowrank", C.c_int64), ("nrow_panels", C.c_int64),
                 ("ncol_panels", C.c_int64), ("factor_bytes", C.c_int64),
-                ("bytes_by_format", C.c_int64 * 4), ("cols_by_format", C.c_int64 * 4),
+                ("bytes_by_format", C.c_int64 * NFMT), ("cols_by_format", C.c_int64 * NFMT),
                 ("scale_bytes", C.c_int64), ("flops", C.c_double), ("datasize", C.c_double),
                 ("index_bytes", C.c_int64), ("one_level", C.c_int32), ("reserved", C.c_int32),
                 ("bytes_rpass", C.c_double), ("bytes_vpass", C.c_double)]

@@ -75,12 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_bv1(const BatchMat* __restrict__ m
     const VPack* pk = M.D.vpacks + w.first + wave;
     const XT* x = static_cast<const XT*>(M.x);
     double* xs = lds + wave * kPackSegs * kXsStride;
-    switch (pk->fmt) {
-    case F64: vpack_fwd<double>(M.D, pk, x, M.part, xs, lane); break;
-    case F32: vpack_fwd<float>(M.D, pk, x, M.part, xs, lane); break;
-    case BF16: vpack_fwd<uint16_t>(M.D, pk, x, M.part, xs, lane); break;
-    default: vpack_fwd<uint8_t>(M.D, pk, x, M.part, xs, lane); break;
-    }
+    HMVM_FMT_SWITCH(pk->fmt, vpack_fwd<T>(M.D, pk, x, M.part, xs, lane))
 }
 
 template <typename XT>

@@ -12,9 +12,6 @@ constexpr int kThreads = 64 * kWaves;
 constexpr int64_t kWholeZBytes = 48 * 1024;   // largest panel z block staged whole in LDS
 constexpr int kXsStride = 65;                 // LDS row stride of the x slices of a V pack
 
-template <typename T> struct IsFp8 { static constexpr bool value = false; };
-template <> struct IsFp8<uint8_t> { static constexpr bool value = true; };
-
 // ---------------------------------------------------------------------------
 // V pass: one wave processes one pack of consecutive segments (lane = group)
 // ---------------------------------------------------------------------------
@@ -36,13 +33,13 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
         if (i < pk.m[j]) xs[j * kXsStride + i] = A(x[pk.x0[j] + i]);
     }
     wave_sync();
-    if (lane >= pk.gofs[nseg]) return;
+    const int ngr = pk.gofs[nseg];                  // groups of the pack = active lanes
     int j = 0;
     while (j + 1 < nseg && lane >= pk.gofs[j + 1]) ++j;
     const int g = lane - pk.gofs[j];
-    const int m = pk.m[j];
-    const int64_t ld = int64_t(pk.ng[j]) * W;
-    const T* base = static_cast<const T*>(P.arena[E::F]) + pk.data_off[j] + int64_t(g) * W;
+    const int m = (lane < ngr) ? pk.m[j] : 0;       // lanes past the pack's groups load nothing
+    const int64_t ldb = int64_t(pk.ng[j]) * 16;                 // bytes per row of the segment
+    const bytep base = elem_addr<T>(P.arena[E::F], pk.data_off[j]) + int64_t(g) * 16;
     const A* xj = xs + j * kXsStride;
     A acc[W];
 #pragma unroll
@@ -51,17 +48,31 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
 #pragma unroll kUnroll
     for (int i = 0; i < m; ++i) {
         A v[W];
-        E::load(base + int64_t(i) * ld, v);
+        E::load(base + int64_t(i) * ldb, v);
         const A xi = xj[i];
 #pragma unroll
         for (int w = 0; w < W; ++w) acc[w] += v[w] * xi;
     }
-    const int64_t vc0 = int64_t(pk.vc0[j]) + int64_t(g) * W;
+    // The lane's W sums are columns lane W .. lane W + W - 1 of the pack.
+    // They go out through LDS (the x slices are no longer read), so that
+    // consecutive lanes then handle consecutive columns: the destination
+    // and scale tables are read and the partials written coalesced, instead
+    // of W strided accesses per lane (one cache line per lane for wide W).
+    static_assert(64 * W * sizeof(A) <= kPackSegs * kXsStride * sizeof(double), "result staging fits the slices");
+    wave_sync();
+    A* res = xs;
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-        const int32_t d = P.vdest[vc0 + w];
-        double v = double(acc[w]);
-        if (IsFp8<T>::value) v *= double(P.vscale[vc0 + w]);
+    for (int w = 0; w < W; ++w) res[lane * W + w] = acc[w];
+    wave_sync();
+    const int ncol = ngr * W;
+    for (int c = lane; c < ncol; c += 64) {
+        const int gr = c / W;
+        int s = 0;
+        while (s + 1 < nseg && gr >= pk.gofs[s + 1]) ++s;
+        const int64_t vc = int64_t(pk.vc0[s]) + (c - pk.gofs[s] * W);
+        const int32_t d = P.vdest[vc];
+        double v = double(res[c]);
+        if (E::kScaled) v *= double(P.vscale[vc]);
         if (d >= 0) Pb[d] = v;
     }
 }
@@ -79,11 +90,26 @@ __device__ inline void chunk_stage(const DevPlan& P, const RChunk& C, double alp
     constexpr int W = E::W;
     A* zs = static_cast<A*>(zbuf);
     const int ncol = C.ngroups * W;
-    for (int j = tid; j < ncol; j += kThreads) {
-        const int32_t s = P.colsrc[C.col0 + j];
-        double z = (s >= 0) ? alpha * double(x[s]) : (s == kPadCol ? 0.0 : t[~s]);
-        if (IsFp8<T>::value) z *= double(P.zscale[C.col0 + j]);
-        zs[j] = A(z);
+    // U columns per thread per step, their index, scale and source loads
+    // issued together (one memory latency per step, not one per column)
+    constexpr int U = 4;
+    for (int j0 = tid; j0 < ncol; j0 += U * kThreads) {
+        int32_t s[U];
+        float sc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * kThreads;
+            s[u] = (j < ncol) ? P.colsrc[C.col0 + j] : kPadCol;
+            sc[u] = (E::kScaled && j < ncol) ? P.zscale[C.col0 + j] : 1.0f;
+        }
+        double z[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) z[u] = (s[u] >= 0) ? alpha * double(x[s[u]]) : (s[u] == kPadCol ? 0.0 : t[~s[u]]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * kThreads;
+            if (j < ncol) zs[j] = A(E::kScaled ? z[u] * double(sc[u]) : z[u]);
+        }
     }
 }
 
@@ -101,8 +127,8 @@ __device__ inline double chunk_compute(const DevPlan& P, const RChunk& C, int m,
 #pragma unroll
     for (int w = 0; w < W; ++w) acc[w] = A(0);
     if (lane < m) {
-        const T* base = static_cast<const T*>(P.arena[E::F]) + C.data_off + int64_t(lane) * W;
-        const int64_t gs = int64_t(m) * W;
+        const bytep base = elem_addr<T>(P.arena[E::F], C.data_off) + int64_t(lane) * 16;
+        const int64_t gs = int64_t(m) * 16;                      // bytes per group
         constexpr int kUnroll = (W >= 8) ? 4 : 8;
 #pragma unroll kUnroll
         for (int g = gb; g < ge; ++g) {
@@ -132,19 +158,9 @@ __device__ inline double rpanel_fwd_chunked(const DevPlan& P, const RowPanel& R,
     for (int ci = 0; ci < R.nchunks; ++ci) {
         const RChunk C = P.rchunks[R.chunk0 + ci];
         __syncthreads();
-        switch (C.fmt) {
-        case F64: chunk_stage<double>(P, C, alpha, x, t, zbuf, tid); break;
-        case F32: chunk_stage<float>(P, C, alpha, x, t, zbuf, tid); break;
-        case BF16: chunk_stage<uint16_t>(P, C, alpha, x, t, zbuf, tid); break;
-        default: chunk_stage<uint8_t>(P, C, alpha, x, t, zbuf, tid); break;
-        }
+        HMVM_FMT_SWITCH(C.fmt, chunk_stage<T>(P, C, alpha, x, t, zbuf, tid))
         __syncthreads();
-        switch (C.fmt) {
-        case F64: acc += chunk_compute<double>(P, C, R.m, zbuf, wave, lane); break;
-        case F32: acc += chunk_compute<float>(P, C, R.m, zbuf, wave, lane); break;
-        case BF16: acc += chunk_compute<uint16_t>(P, C, R.m, zbuf, wave, lane); break;
-        default: acc += chunk_compute<uint8_t>(P, C, R.m, zbuf, wave, lane); break;
-        }
+        HMVM_FMT_SWITCH(C.fmt, acc += chunk_compute<T>(P, C, R.m, zbuf, wave, lane))
     }
     return acc;
 }

@@ -47,12 +47,7 @@ __global__ __launch_bounds__(kThreads) void k_vfwd(DevPlan P, const XT* __restri
     const int64_t pi = int64_t(blockIdx.x) * kWaves + wave;
     if (pi >= P.nvpacks) return;
     const VPack* pk = P.vpacks + pi;
-    switch (pk->fmt) {
-    case F64: vpack_fwd<double>(P, pk, x, Pb, xs[wave], lane); break;
-    case F32: vpack_fwd<float>(P, pk, x, Pb, xs[wave], lane); break;
-    case BF16: vpack_fwd<uint16_t>(P, pk, x, Pb, xs[wave], lane); break;
-    default: vpack_fwd<uint8_t>(P, pk, x, Pb, xs[wave], lane); break;
-    }
+    HMVM_FMT_SWITCH(pk->fmt, vpack_fwd<T>(P, pk, x, Pb, xs[wave], lane))
 }
 
 // stage 1: tmp[s] = sum of kSegLen consecutive partials of entry se[s]
@@ -105,23 +100,13 @@ __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, cons
         for (int ci = 0; ci < R.nchunks; ++ci) {
             const RChunk C = P.rchunks[R.chunk0 + ci];
             void* zb = zdyn + C.zoff;
-            switch (C.fmt) {
-            case F64: chunk_stage<double>(P, C, alpha, x, t, zb, tid); break;
-            case F32: chunk_stage<float>(P, C, alpha, x, t, zb, tid); break;
-            case BF16: chunk_stage<uint16_t>(P, C, alpha, x, t, zb, tid); break;
-            default: chunk_stage<uint8_t>(P, C, alpha, x, t, zb, tid); break;
-            }
+            HMVM_FMT_SWITCH(C.fmt, chunk_stage<T>(P, C, alpha, x, t, zb, tid))
         }
         __syncthreads();
         for (int ci = 0; ci < R.nchunks; ++ci) {
             const RChunk C = P.rchunks[R.chunk0 + ci];
             const void* zb = zdyn + C.zoff;
-            switch (C.fmt) {
-            case F64: acc += chunk_compute<double>(P, C, R.m, zb, wave, lane); break;
-            case F32: acc += chunk_compute<float>(P, C, R.m, zb, wave, lane); break;
-            case BF16: acc += chunk_compute<uint16_t>(P, C, R.m, zb, wave, lane); break;
-            default: acc += chunk_compute<uint8_t>(P, C, R.m, zb, wave, lane); break;
-            }
+            HMVM_FMT_SWITCH(C.fmt, acc += chunk_compute<T>(P, C, R.m, zb, wave, lane))
         }
     } else {
         acc = rpanel_fwd_chunked(P, R, alpha, x, t, zdyn, tid, wave, lane);
@@ -150,11 +135,15 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const RtItem* __re
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
-    constexpr int K = (sizeof(A) == 8) ? 8 : 16;    // columns reduced together (16 registers live)
+    // K columns reduced together (16 registers live; 8 for FP64 and for
+    // W = 5, 6, where 16 would leave a third of the slots empty); a
+    // reduction covers SG whole groups and pads the K - SG W slots left
+    // with zeros, a step NL = a multiple of SG groups
+    constexpr int K = (sizeof(A) == 8 || W == 5 || W == 6) ? 8 : 16;
     constexpr int SG = K / W;                       // groups per reduction
-    constexpr int NL = kRtLoads;                    // groups per step
+    constexpr int NL = (kRtLoads / SG) * SG;        // groups per step
     constexpr int LPC = 64 / K;                     // lanes holding one column sum
-    static_assert(SG >= 1 && NL % SG == 0, "a reduction covers whole groups, a step whole reductions");
+    static_assert(SG >= 1 && NL >= SG, "a reduction covers whole groups, a step whole reductions");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
     if (idx >= n) return;
@@ -165,8 +154,8 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const RtItem* __re
     const int m = R.m;
     const A xi = (lane < m) ? A(x[R.row0 + lane]) : A(0);
     const bool on = lane < m;
-    const T* base = static_cast<const T*>(P.arena[E::F]) + C.data_off + int64_t(on ? lane : 0) * W;
-    const int64_t gs = int64_t(m) * W;
+    const bytep base = elem_addr<T>(P.arena[E::F], C.data_off) + int64_t(on ? lane : 0) * 16;
+    const int64_t gs = int64_t(m) * 16;                 // bytes per group
     const int ncol = C.ngroups * W;
 #pragma unroll 1
     for (int g0 = wi.g0; g0 < gend; g0 += NL) {
@@ -184,13 +173,16 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const RtItem* __re
 #pragma unroll
                 for (int w = 0; w < W; ++w) v[gg * W + w] = u[w] * xi;     // xi = 0 on rows past m
             }
+#pragma unroll
+            for (int k = SG * W; k < K; ++k) v[k] = A(0);
             const A s = xreduce<K>(v, lane);
-            const int col = (g0 + sb * SG) * W + lane / LPC;
-            if ((lane % LPC) == 0 && col < min(ncol, gend * W)) {
+            const int slot = lane / LPC;
+            const int col = (g0 + sb * SG) * W + slot;
+            if ((lane % LPC) == 0 && slot < SG * W && col < min(ncol, gend * W)) {
                 const int64_t gc = C.col0 + col;
                 const int32_t d = P.rdest[gc];
                 double sv = double(s);
-                if (IsFp8<T>::value) sv *= double(P.zscale[gc]);
+                if (E::kScaled) sv *= double(P.zscale[gc]);
                 if (d >= 0) Q[d] = sv;
             }
         }
@@ -223,11 +215,11 @@ __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VIte
         const int64_t vc = C.vcol0[E::F] + int64_t(it.g0 + g) * W + w;
         const int32_t e = on ? P.vsrc[vc] : -1;
         double v = (e >= 0) ? s[e] : 0.0;
-        if (IsFp8<T>::value && on) v *= double(P.vscale[vc]);
+        if (E::kScaled && on) v *= double(P.vscale[vc]);
         sv[w] = v;
     }
-    const int64_t ld = int64_t(it.ng) * W;
-    const T* base = static_cast<const T*>(P.arena[E::F]) + it.data_off + int64_t(g) * W;
+    const int64_t ldb = int64_t(it.ng) * 16;            // bytes per row of the item
+    const bytep base = elem_addr<T>(P.arena[E::F], it.data_off) + int64_t(g) * 16;
     constexpr int NL = kVtLoads;
     static_assert(NL % kTRows == 0, "a step covers whole row blocks");
 #pragma unroll 1
@@ -237,7 +229,7 @@ __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VIte
         u32x4 raw[NL];
 #pragma unroll
         for (int r = 0; r < NL; ++r)
-            raw[r] = (i0 + r < C.m) ? raw16(base + int64_t(i0 + r) * ld) : u32x4{0u, 0u, 0u, 0u};
+            raw[r] = (i0 + r < C.m) ? raw16(base + int64_t(i0 + r) * ldb) : u32x4{0u, 0u, 0u, 0u};
 #pragma unroll
         for (int sb = 0; sb < NL / kTRows; ++sb) {
             double d[kTRows];
@@ -266,12 +258,7 @@ __global__ __launch_bounds__(kThreads) void k_vtrn(DevPlan P, double alpha, cons
     red[wave][lane] = 0.0;          // the wave's row sums; only this wave touches its row
     for (int j = wave; j < C.nitems; j += kWaves) {
         const VItem it = P.vitems[C.item0 + j];
-        switch (it.fmt) {
-        case F64: item_vtrn<double>(P, C, it, s, red[wave], lane); break;
-        case F32: item_vtrn<float>(P, C, it, s, red[wave], lane); break;
-        case BF16: item_vtrn<uint16_t>(P, C, it, s, red[wave], lane); break;
-        default: item_vtrn<uint8_t>(P, C, it, s, red[wave], lane); break;
-        }
+        HMVM_FMT_SWITCH(it.fmt, item_vtrn<T>(P, C, it, s, red[wave], lane))
     }
     __syncthreads();
     if (wave == 0 && lane < C.m) {
@@ -327,7 +314,10 @@ __global__ __launch_bounds__(kThreads) void k_grad_v(DevPlan P, const XT* __rest
     const int64_t item = int64_t(blockIdx.x) * kWaves + wave;
     if (item >= P.nvitems) return;
     const VItem it = P.vitems[item];
-    double* gV = gA.p[it.fmt];
+    double* gV = nullptr;             // selected without dynamic indexing (which would go through scratch)
+#pragma unroll
+    for (int f = 0; f < NFMT; ++f)
+        if (it.fmt == f) gV = gA.p[f];
     if (!gV || lane >= it.ng) return;
     const ColPanel C = P.cp[it.q];
     const int W = fmt_width(it.fmt);
@@ -352,11 +342,11 @@ struct hmvm_plan {
     int device = -1;
     DevPlan dp{};
     std::vector<void*> allocs;
-    void* arena_dev[NFMT] = {nullptr, nullptr, nullptr, nullptr};
+    void* arena_dev[NFMT] = {};
     // R work items (chunk, first group; at most kRtSpan groups) grouped by
     // format, for the transposed and gradient launches
     const RtItem* rtitems_dev = nullptr;
-    int64_t rtitem_start[NFMT + 1] = {0, 0, 0, 0, 0};
+    int64_t rtitem_start[NFMT + 1] = {};
 };
 
 namespace {
@@ -428,11 +418,13 @@ int upload_plan(hmvm_plan* p) {
     p->rtitem_start[NFMT] = int64_t(items.size());
     if ((rc = upload(p, items, &p->rtitems_dev)) != HMVM_OK) return rc;
     // arenas
-    const void* a[NFMT] = {nullptr, nullptr, nullptr, nullptr};
-    if ((rc = upload(p, L.a64, reinterpret_cast<const double**>(&a[0]))) != HMVM_OK) return rc;
-    if ((rc = upload(p, L.a32, reinterpret_cast<const float**>(&a[1]))) != HMVM_OK) return rc;
-    if ((rc = upload(p, L.a16, reinterpret_cast<const uint16_t**>(&a[2]))) != HMVM_OK) return rc;
-    if ((rc = upload(p, L.a8, reinterpret_cast<const uint8_t**>(&a[3]))) != HMVM_OK) return rc;
+    const void* a[NFMT] = {};
+    if ((rc = upload(p, L.a64, reinterpret_cast<const double**>(&a[F64]))) != HMVM_OK) return rc;
+    if ((rc = upload(p, L.a32, reinterpret_cast<const float**>(&a[F32]))) != HMVM_OK) return rc;
+    if ((rc = upload(p, L.a16, reinterpret_cast<const uint16_t**>(&a[BF16]))) != HMVM_OK) return rc;
+    if ((rc = upload(p, L.a8, reinterpret_cast<const uint8_t**>(&a[FP8]))) != HMVM_OK) return rc;
+    for (int f = kFirstBfp; f < NFMT; ++f)
+        if ((rc = upload(p, L.abfp[f - kFirstBfp], reinterpret_cast<const uint32_t**>(&a[f]))) != HMVM_OK) return rc;
     for (int f = 0; f < NFMT; ++f) { D.arena[f] = a[f]; p->arena_dev[f] = const_cast<void*>(a[f]); }
     D.nrp = int64_t(L.rp.size());
     D.ncp = int64_t(L.cp.size());
@@ -447,10 +439,7 @@ int upload_plan(hmvm_plan* p) {
     D.nrows = L.nrows;
     D.ncols = L.ncols;
     // the host copies of the arenas are no longer needed on the device path
-    std::vector<double>().swap(L.a64);
-    std::vector<float>().swap(L.a32);
-    std::vector<uint16_t>().swap(L.a16);
-    std::vector<uint8_t>().swap(L.a8);
+    L.release_arenas();
     return HMVM_OK;
 }
 
@@ -512,12 +501,7 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
             if (n == 0) continue;
             const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
             const unsigned nb = blocks_for(n, kWaves);
-            switch (f) {
-            case F64: k_rtrn<double, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part); break;
-            case F32: k_rtrn<float, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part); break;
-            case BF16: k_rtrn<uint16_t, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part); break;
-            default: k_rtrn<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part); break;
-            }
+            HMVM_FMT_SWITCH(f, k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part))
         }
         if (ev) (void)hipEventRecord(ev[1], st);
         launch_psum(D, true, part, tmp, t, alpha, st);
@@ -759,7 +743,7 @@ int hmvm_tlr_apply(const hmvm_plan* p, double alpha, int op, const void* x, void
     return hmvm_apply(p, alpha, op, x, 0, y, 0, 1, vdt, stream);
 }
 
-int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, void* grad_arena[4], int vdt,
+int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, void* grad_arena[NFMT], int vdt,
               void* stream) {
     int rc = check_apply_args(p, HMVM_OP_NORMAL, x, ybar, 1, vdt);
     if (rc != HMVM_OK) return rc;
@@ -789,12 +773,7 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
             if (n == 0) continue;
             const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
             const unsigned nb = blocks_for(n, kWaves);
-            switch (f) {
-            case F64: k_rtrn<double, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb); break;
-            case F32: k_rtrn<float, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb); break;
-            case BF16: k_rtrn<uint16_t, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb); break;
-            default: k_rtrn<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb); break;
-            }
+            HMVM_FMT_SWITCH(f, k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb))
         }
         launch_psum(D, false, Pb, tmp, t, 1.0, st);
         launch_psum(D, true, Qb, tmp, s, 1.0, st);
@@ -807,12 +786,7 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
                 if (nr > 0) {
                     const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
                     const unsigned nb = blocks_for(nr, kWaves);
-                    switch (f) {
-                    case F64: k_grad_r<double, XT><<<nb, kThreads, 0, st>>>(D, ids, nr, xx, yb, t, gA); break;
-                    case F32: k_grad_r<float, XT><<<nb, kThreads, 0, st>>>(D, ids, nr, xx, yb, t, gA); break;
-                    case BF16: k_grad_r<uint16_t, XT><<<nb, kThreads, 0, st>>>(D, ids, nr, xx, yb, t, gA); break;
-                    default: k_grad_r<uint8_t, XT><<<nb, kThreads, 0, st>>>(D, ids, nr, xx, yb, t, gA); break;
-                    }
+                    HMVM_FMT_SWITCH(f, k_grad_r<T, XT><<<nb, kThreads, 0, st>>>(D, ids, nr, xx, yb, t, gA))
                 }
             }
             GradPtrs gp;
@@ -828,7 +802,7 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
 
 int hmvm_plan_arena_elems(const hmvm_plan* p, int fmt, int64_t* elems) {
     if (!p || !elems || fmt < 0 || fmt >= NFMT) return HMVM_ERR_INVALID_ARG;
-    *elems = p->device >= 0 ? p->L.bytes_by_format[fmt] / fmt_bytes(fmt) : p->L.arena_elems(fmt);
+    *elems = p->L.arena_elems(fmt);
     return HMVM_OK;
 }
 
@@ -837,10 +811,7 @@ int hmvm_plan_get_arena(const hmvm_plan* p, int fmt, void* dst, void* stream) {
     const int64_t bytes = p->L.bytes_by_format[fmt];
     if (bytes == 0) return HMVM_OK;
     if (p->device < 0) {
-        const HostLayout& L = p->L;
-        const void* src = fmt == F64 ? (const void*)L.a64.data() : fmt == F32 ? (const void*)L.a32.data()
-                        : fmt == BF16 ? (const void*)L.a16.data() : (const void*)L.a8.data();
-        std::memcpy(dst, src, size_t(bytes));
+        std::memcpy(dst, p->L.arena_host(fmt), size_t(bytes));
         return HMVM_OK;
     }
     DeviceGuard g(p->device);
@@ -860,7 +831,7 @@ int hmvm_plan_set_arena(hmvm_plan* p, int fmt, const void* src, void* stream) {
                    hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
 }
 
-int hmvm_plan_unpack_block(const hmvm_plan* p, int64_t block, const void* const host_arenas[4], int as_f64,
+int hmvm_plan_unpack_block(const hmvm_plan* p, int64_t block, const void* const host_arenas[NFMT], int as_f64,
                            double* a, double* b) {
     if (!p || !host_arenas) return HMVM_ERR_INVALID_ARG;
     return unpack_block(p->L, block, host_arenas, as_f64 != 0, a, b);

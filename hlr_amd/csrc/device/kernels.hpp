@@ -63,15 +63,20 @@ __device__ inline u32x4 raw16(const void* p) {
     return __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
 }
 
+// Factor rows are addressed by value offsets (multiples of W): the row of
+// offset `off` starts at byte (off / W) * 16 of its format's arena.
+using bytep = const unsigned char*;
+
 template <> struct Elem<double> {
     static constexpr int F = F64;
     static constexpr int W = 2;
+    static constexpr bool kScaled = false;
     using Acc = double;
     __device__ static inline void decode(u32x4 v, double* out) {
         out[0] = __hiloint2double(int(v.y), int(v.x));
         out[1] = __hiloint2double(int(v.w), int(v.z));
     }
-    __device__ static inline void load(const double* p, double* out) {
+    __device__ static inline void load(bytep p, double* out) {
         const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p));
         out[0] = v.x; out[1] = v.y;
     }
@@ -80,12 +85,13 @@ template <> struct Elem<double> {
 template <> struct Elem<float> {
     static constexpr int F = F32;
     static constexpr int W = 4;
+    static constexpr bool kScaled = false;
     using Acc = float;
     __device__ static inline void decode(u32x4 v, float* out) {
         out[0] = __uint_as_float(v.x); out[1] = __uint_as_float(v.y);
         out[2] = __uint_as_float(v.z); out[3] = __uint_as_float(v.w);
     }
-    __device__ static inline void load(const float* p, float* out) {
+    __device__ static inline void load(bytep p, float* out) {
         const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
         out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
     }
@@ -94,6 +100,7 @@ template <> struct Elem<float> {
 template <> struct Elem<uint16_t> {   // BF16
     static constexpr int F = BF16;
     static constexpr int W = 8;
+    static constexpr bool kScaled = false;
     using Acc = float;
     __device__ static inline void decode(u32x4 v, float* out) {
         const unsigned w[4] = {v.x, v.y, v.z, v.w};
@@ -103,12 +110,13 @@ template <> struct Elem<uint16_t> {   // BF16
             out[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
         }
     }
-    __device__ static inline void load(const uint16_t* p, float* out) { decode(raw16(p), out); }
+    __device__ static inline void load(bytep p, float* out) { decode(raw16(p), out); }
 };
 
 template <> struct Elem<uint8_t> {    // FP8 E4M3 (OCP)
     static constexpr int F = FP8;
     static constexpr int W = 16;
+    static constexpr bool kScaled = true;
     using Acc = float;
     __device__ static inline void decode(u32x4 v, float* out) {
         const unsigned w[4] = {v.x, v.y, v.z, v.w};
@@ -119,8 +127,50 @@ template <> struct Elem<uint8_t> {    // FP8 E4M3 (OCP)
             out[4 * i] = lo.x; out[4 * i + 1] = lo.y; out[4 * i + 2] = hi.x; out[4 * i + 3] = hi.y;
         }
     }
-    __device__ static inline void load(const uint8_t* p, float* out) { decode(raw16(p), out); }
+    __device__ static inline void load(bytep p, float* out) { decode(raw16(p), out); }
 };
+
+// block floating point with NB-bit values (tag type; the arena holds packed rows)
+template <int NB> struct Bfp {};
+constexpr int bfp_format(int nb) {
+    return nb == 8 ? B8 : nb == 10 ? B10 : nb == 12 ? B12 : nb == 16 ? B16 : nb == 21 ? B21 : nb == 25 ? B25 : -1;
+}
+
+template <int NB> struct Elem<Bfp<NB>> {
+    static constexpr int F = bfp_format(NB);
+    static_assert(F >= kFirstBfp, "no storage format with this width");
+    static constexpr int W = 128 / NB;
+    static constexpr bool kScaled = true;
+    using Acc = float;
+    // |q| < 2^24, so the conversion is exact
+    __device__ static inline void decode(u32x4 v, float* out) {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < W; ++j) out[j] = float(bfp_field<NB>(w, j));
+    }
+    __device__ static inline void load(bytep p, float* out) { decode(raw16(p), out); }
+};
+
+// start of the row holding value offset `off` (a multiple of W) in an arena
+template <typename T>
+__device__ inline bytep elem_addr(const void* arena, int64_t off) {
+    return static_cast<bytep>(arena) + (off / Elem<T>::W) * 16;
+}
+
+// Runs the statements with T bound to the element type of storage format f.
+#define HMVM_FMT_SWITCH(f, ...)                                      \
+    switch (f) {                                                     \
+    case F64: { using T = double; __VA_ARGS__; } break;              \
+    case F32: { using T = float; __VA_ARGS__; } break;               \
+    case BF16: { using T = uint16_t; __VA_ARGS__; } break;           \
+    case FP8: { using T = uint8_t; __VA_ARGS__; } break;             \
+    case B8: { using T = Bfp<8>; __VA_ARGS__; } break;               \
+    case B10: { using T = Bfp<10>; __VA_ARGS__; } break;             \
+    case B12: { using T = Bfp<12>; __VA_ARGS__; } break;             \
+    case B16: { using T = Bfp<16>; __VA_ARGS__; } break;             \
+    case B21: { using T = Bfp<21>; __VA_ARGS__; } break;             \
+    default: { using T = Bfp<25>; __VA_ARGS__; } break;              \
+    }
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

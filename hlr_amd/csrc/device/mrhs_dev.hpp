@@ -20,7 +20,13 @@ constexpr int kMaxRp = 64;           // right-hand sides per pass (tiles of 16)
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-template <typename T> struct Mm;
+// every format but FP64 is decoded to FP32 and uses the FP32 matrix cores
+template <typename T> struct Mm {
+    using A = float; using V4 = f4;
+    static constexpr int F = Elem<T>::F; static constexpr int W = Elem<T>::W;
+    __device__ static inline V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static inline int drow(int kk, int e) { return 4 * kk + e; }
+};
 template <> struct Mm<double> {
     using A = double; using V4 = d4;
     static constexpr int F = F64; static constexpr int W = 2;
@@ -28,31 +34,14 @@ template <> struct Mm<double> {
     // tile row of element e held by lanes with lane / 16 == kk
     __device__ static inline int drow(int kk, int e) { return kk + 4 * e; }
 };
-template <> struct Mm<float> {
-    using A = float; using V4 = f4;
-    static constexpr int F = F32; static constexpr int W = 4;
-    __device__ static inline V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    __device__ static inline int drow(int kk, int e) { return 4 * kk + e; }
-};
-template <> struct Mm<uint16_t> {
-    using A = float; using V4 = f4;
-    static constexpr int F = BF16; static constexpr int W = 8;
-    __device__ static inline V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    __device__ static inline int drow(int kk, int e) { return 4 * kk + e; }
-};
-template <> struct Mm<uint8_t> {
-    using A = float; using V4 = f4;
-    static constexpr int F = FP8; static constexpr int W = 16;
-    __device__ static inline V4 mma(float a, float b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    __device__ static inline int drow(int kk, int e) { return 4 * kk + e; }
-};
 
 // MFMA steps whose factor loads are issued together: 8 for FP64 / FP32,
 // fewer for the narrow formats, whose loads each feed more MFMAs
-template <int W> struct MmSteps { static constexpr int value = (W <= 4) ? 8 : (W == 8 ? 4 : 2); };
+template <int W> struct MmSteps { static constexpr int value = (W <= 4) ? 8 : (W <= 8 ? 4 : 2); };
 
-template <typename T> struct IsF8 { static constexpr bool v = false; };
-template <> struct IsF8<uint8_t> { static constexpr bool v = true; };
+// V pass accumulator sets per segment pass: the largest divisor of W that
+// is at most 5 (W / JB passes over the segment)
+constexpr int mm_jb(int W) { return W % 5 == 0 ? 5 : W % 4 == 0 ? 4 : W % 3 == 0 ? 3 : W % 2 == 0 ? 2 : 1; }
 
 // ---------------------------------------------------------------------------
 // V pass: one workgroup per segment, out(c, r) = sum_i V(i, c) X(x0 + i, r),
@@ -102,7 +91,7 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
     using A = typename M::A;
     using E = Elem<T>;
     constexpr int W = M::W;
-    constexpr int JB = (W < 4) ? W : 4;
+    constexpr int JB = mm_jb(W);
     constexpr int NB = MmSteps<W>::value;
     const ColPanel C = P.cp[sg.q];
     const int m = C.m;
@@ -120,8 +109,8 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
     const int n = lane & 15, kk = lane >> 4;
     const int gl = 16 * wave + n;
     const bool gon = gl < sg.ng;
-    const int64_t ld = int64_t(sg.ng) * W;
-    const T* base = static_cast<const T*>(P.arena[M::F]) + sg.data_off + int64_t(gon ? gl : 0) * W;
+    const int64_t ldb = int64_t(sg.ng) * 16;                // bytes per row of the segment
+    const bytep base = elem_addr<T>(P.arena[M::F], sg.data_off) + int64_t(gon ? gl : 0) * 16;
     const int64_t vc0 = C.vcol0[M::F] + int64_t(sg.g0) * W;
     for (int jp = 0; jp < W; jp += JB) {
         typename M::V4 acc[JB][NTM];
@@ -135,7 +124,7 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int row = 4 * (s0 + b) + kk;
-                raw[b] = (gon && row < m) ? raw16(base + int64_t(row) * ld) : u32x4{0u, 0u, 0u, 0u};
+                raw[b] = (gon && row < m) ? raw16(base + int64_t(row) * ldb) : u32x4{0u, 0u, 0u, 0u};
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -160,7 +149,7 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
                 const int64_t vc = vc0 + int64_t(gl) * W + jp + j;
                 const int32_t d = P.vdest[vc];
                 if (d < 0) continue;
-                const double sc = IsF8<T>::v ? double(P.vscale[vc]) : 1.0;
+                const double sc = E::kScaled ? double(P.vscale[vc]) : 1.0;
 #pragma unroll
                 for (int nt = 0; nt < NTM; ++nt) {
                     if (nt < NT) {
@@ -181,12 +170,7 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
 template <int NTM>
 __device__ inline void vseg_mm_dispatch(const DevPlan& P, const VItem& sg, const double* __restrict__ XTb,
                                         int R, double* __restrict__ Pb, double* xs, int tid, int wave, int lane) {
-    switch (sg.fmt) {
-    case F64: vseg_mm_wg<NTM, double>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
-    case F32: vseg_mm_wg<NTM, float>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
-    case BF16: vseg_mm_wg<NTM, uint16_t>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
-    default: vseg_mm_wg<NTM, uint8_t>(P, sg, XTb, R, Pb, xs, tid, wave, lane); break;
-    }
+    HMVM_FMT_SWITCH(sg.fmt, vseg_mm_wg<NTM, T>(P, sg, XTb, R, Pb, xs, tid, wave, lane))
 }
 
 // ---------------------------------------------------------------------------
@@ -232,7 +216,11 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     using A = typename M::A;
     using E = Elem<T>;
     constexpr int W = M::W;
-    constexpr int GB = 64 / W;                           // groups per staged block of 64 columns
+    // groups per staged block of at most 64 columns: a multiple of the 4
+    // groups of one MFMA step, so that no step reads past the staged columns
+    constexpr int GB = (64 / W) & ~3;
+    constexpr int SC = GB * W;                           // staged columns per block
+    static_assert(GB >= 4, "a staged block holds at least one MFMA step");
     constexpr int NB = MmSteps<W>::value;
     constexpr int RS = 16 * NTM;                         // staged rhs slots (constant divisor)
     A* zs = static_cast<A*>(zraw);                       // [64][ZS]: column, rhs
@@ -241,10 +229,10 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     const int ncol = Ch.ngroups * W;
     const int row = 16 * wave + (lane & 15);
     const int kk = lane >> 4, n = lane & 15;
-    const T* base = static_cast<const T*>(P.arena[M::F]) + Ch.data_off + int64_t(row < m ? row : 0) * W;
+    const bytep base = elem_addr<T>(P.arena[M::F], Ch.data_off) + int64_t(row < m ? row : 0) * 16;
     for (int gb0 = 0; gb0 < Ch.ngroups; gb0 += GB) {
         __syncthreads();
-        for (int idx = tid; idx < 64 * RS; idx += kThr) {
+        for (int idx = tid; idx < SC * RS; idx += kThr) {
             const int c = idx / RS, r = idx - c * RS;    // rhs fastest: contiguous reads of XT and t
             if (r >= Rp) continue;
             const int col = gb0 * W + c;
@@ -252,7 +240,7 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
             if (col < ncol && r < R) {
                 const int32_t s = P.colsrc[Ch.col0 + col];
                 z = (s >= 0) ? alpha * XTb[int64_t(s) * R + r] : (s == kPadCol ? 0.0 : t[int64_t(~s) * R + r]);
-                if (IsF8<T>::v) z *= double(P.zscale[Ch.col0 + col]);
+                if (E::kScaled) z *= double(P.zscale[Ch.col0 + col]);
             }
             zs[c * ZS + r] = A(z);
         }
@@ -264,7 +252,7 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int g = g0 + 4 * b + kk;
-                raw[b] = (row < m && g < gend) ? raw16(base + int64_t(g) * m * W) : u32x4{0u, 0u, 0u, 0u};
+                raw[b] = (row < m && g < gend) ? raw16(base + int64_t(g) * m * 16) : u32x4{0u, 0u, 0u, 0u};
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -285,6 +273,14 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     }
 }
 
+// the accumulators of a format: FP64 tiles for FP64, FP32 tiles otherwise
+template <typename T> struct RAcc {
+    template <int N> __device__ static inline auto& pick(d4 (&)[N], f4 (&a32)[N]) { return a32; }
+};
+template <> struct RAcc<double> {
+    template <int N> __device__ static inline auto& pick(d4 (&a64)[N], f4 (&)[N]) { return a64; }
+};
+
 // whole row panel, Y(:, 0..R) += R_p Z; all kThr threads call it
 template <int NTM, typename YT>
 __device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double alpha, const double* __restrict__ XTb,
@@ -297,12 +293,8 @@ __device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double al
     for (int i = 0; i < NTM; ++i) { acc64[i] = d4{0, 0, 0, 0}; acc32[i] = f4{0, 0, 0, 0}; }
     for (int ci = 0; ci < Rw.nchunks; ++ci) {
         const RChunk Ch = P.rchunks[Rw.chunk0 + ci];
-        switch (Ch.fmt) {
-        case F64: rchunk_mm<NTM, double>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc64, tid, wave, lane); break;
-        case F32: rchunk_mm<NTM, float>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
-        case BF16: rchunk_mm<NTM, uint16_t>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
-        default: rchunk_mm<NTM, uint8_t>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf, acc32, tid, wave, lane); break;
-        }
+        HMVM_FMT_SWITCH(Ch.fmt, rchunk_mm<NTM, T>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf,
+                                                   RAcc<T>::pick(acc64, acc32), tid, wave, lane))
     }
     // bring the FP64 tiles into the FP32 element order (row 4 kk + e), then
     // one writer per output element: row 16 wave + 4 (lane / 16) + e

@@ -30,7 +30,16 @@
 
 namespace hmvm {
 
-enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, NFMT = 4 };
+// Storage formats of factor columns.  F64, F32, BF16 and FP8 (E4M3) hold
+// W = 16 / sizeof(element) values per 16-byte row.  B8 .. B25 are block
+// floating point: NB-bit two's-complement integers q, W = floor(128 / NB) of
+// them per 16-byte row (value j in bits [j NB, j NB + NB) of the row read as
+// a little-endian 128-bit integer), each standing for q * 2^k with one
+// power-of-two scale 2^k per panel column (as FP8's scale).
+enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, B8 = 4, B10 = 5, B12 = 6, B16 = 7, B21 = 8, B25 = 9,
+                     NFMT = 10 };
+constexpr int kFirstBfp = B8;
+constexpr int kNumBfp = NFMT - kFirstBfp;
 
 constexpr int kPanel = 64;          // rows per panel = lanes per wave
 constexpr int kWaves = 4;           // waves per workgroup
@@ -38,8 +47,26 @@ constexpr int kChunkGroups = 64 * kWaves;   // column groups per R chunk (one pe
 constexpr int kItemGroups = 64;     // column groups per V item (one per lane of a wave)
 constexpr int32_t kPadCol = INT32_MIN;
 
-HMVM_HD inline int fmt_bytes(int f) { return f == F64 ? 8 : f == F32 ? 4 : f == BF16 ? 2 : 1; }
-HMVM_HD inline int fmt_width(int f) { return 16 / fmt_bytes(f); }
+// bits per stored value, values per 16-byte row
+HMVM_HD constexpr int fmt_bits(int f) {
+    return f == F64 ? 64 : f == F32 ? 32 : f == BF16 ? 16 : f == FP8 ? 8 : f == B8 ? 8 : f == B10 ? 10
+         : f == B12 ? 12 : f == B16 ? 16 : f == B21 ? 21 : 25;
+}
+HMVM_HD constexpr int fmt_width(int f) { return 128 / fmt_bits(f); }
+HMVM_HD constexpr bool fmt_is_bfp(int f) { return f >= kFirstBfp; }
+// formats whose panel columns carry a power-of-two scale
+HMVM_HD constexpr bool fmt_scaled(int f) { return f >= FP8; }
+// bytes of `values` stored values (a multiple of the format's width)
+HMVM_HD constexpr int64_t fmt_store_bytes(int f, int64_t values) { return values / fmt_width(f) * 16; }
+
+// NB-bit field j of a 16-byte row w[0..3] (little-endian words), sign-extended
+template <int NB>
+HMVM_HD inline int32_t bfp_field(const uint32_t (&w)[4], int j) {
+    const int bit = j * NB, word = bit >> 5, off = bit & 31;
+    uint32_t u = w[word] >> off;
+    if (off + NB > 32) u |= w[word + 1] << (32 - off);
+    return int32_t(u << (32 - NB)) >> (32 - NB);
+}
 
 struct RowPanel {
     int64_t row0;       // first global row

@@ -8,6 +8,15 @@
 // and per-column scaling as in afl::compress_lr (compress/afl.hh:818-885);
 // here m_l selects the narrowest hardware format with at least m_l mantissa
 // bits (FP8-E4M3: 3, BF16: 7, FP32: 23, FP64: 52) instead of a bitstream.
+//
+// HMVM_PREC_BFP stores columns as block floating point, as bfp::compress_lr
+// (compress/bfp.hh:765-860) does with one scale per column: here the scale
+// is a power of two per panel column and the value width NB comes from a
+// small set of packings (8, 10, 12, 16, 21, 25 bits).  The width of a
+// column is the narrowest whose stored U and V columns are both within
+// 2^-(m_l+1) of the FP64 columns in the 2-norm -- the error a rounding
+// hardware format with m_l mantissa bits guarantees -- so the VALR error
+// budget holds as for the hardware formats.
 
 #include "plan_host.hpp"
 
@@ -69,7 +78,82 @@ float decode_bf16(uint16_t c) {
     return f;
 }
 
+// Step exponent k of a panel column of NB-bit values: the smallest k with
+// max|v| <= (2^(NB-1) - 1) 2^k, kept in FP32's normal range [-126, 127]
+// (the scale 2^k is stored as a float).  0 for an all-zero column.
+int bfp_exp(double maxabs, int nb) {
+    if (!(maxabs > 0) || !std::isfinite(maxabs)) return 0;
+    const double qmax = std::ldexp(1.0, nb - 1) - 1.0;
+    int k = int(std::ceil(std::log2(maxabs / qmax)));
+    while (maxabs > qmax * std::ldexp(1.0, k)) ++k;          // guard the log2 rounding
+    while (maxabs <= qmax * std::ldexp(1.0, k - 1)) --k;
+    return std::min(127, std::max(-126, k));
+}
+
+// q = v / 2^k rounded to nearest (ties to even), clamped to +-(2^(NB-1) - 1)
+int32_t bfp_quant(double v, int k, int nb) {
+    const double qmax = std::ldexp(1.0, nb - 1) - 1.0;
+    double q = std::nearbyint(std::ldexp(v, -k));
+    if (!(q <= qmax)) q = (q < 0) ? -qmax : qmax;        // also catches NaN (never stored)
+    if (q < -qmax) q = -qmax;
+    return int32_t(q);
+}
+
 namespace {
+
+// NB of each block floating point format, narrowest first
+constexpr int kBfpBits[kNumBfp] = {8, 10, 12, 16, 21, 25};
+static_assert(fmt_bits(B8) == 8 && fmt_bits(B10) == 10 && fmt_bits(B12) == 12 && fmt_bits(B16) == 16 &&
+              fmt_bits(B21) == 21 && fmt_bits(B25) == 25, "format table");
+
+// sum of squared errors of one panel column stored with NB-bit values
+double bfp_err2(const double* src, int64_t m, int nb) {
+    double mx = 0;
+    for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
+    const int k = bfp_exp(mx, nb);
+    double e2 = 0;
+    for (int64_t i = 0; i < m; ++i) {
+        const double d = std::ldexp(double(bfp_quant(src[i], k, nb)), k) - src[i];
+        e2 += d * d;
+    }
+    return e2;
+}
+
+struct Piece {            // one panel column: m values from src
+    const double* src;
+    int64_t m;
+};
+
+// the narrowest block floating point format whose stored pieces of every
+// column in `cols` are within 2^-(rate+1) of the column in the 2-norm; else
+// FP32 when rate <= 23 (its rounding meets the same bound), else FP64
+int bfp_choose(const std::vector<std::vector<Piece>>& cols, int rate) {
+    std::vector<double> thr(cols.size());
+    for (size_t c = 0; c < cols.size(); ++c) {
+        double n2 = 0;
+        for (const Piece& pc : cols[c])
+            for (int64_t i = 0; i < pc.m; ++i) n2 += pc.src[i] * pc.src[i];
+        thr[c] = std::ldexp(n2, -2 * (rate + 1));
+    }
+    for (int f = 0; f < kNumBfp; ++f) {
+        bool ok = true;
+        for (size_t c = 0; c < cols.size() && ok; ++c) {
+            double e2 = 0;
+            for (const Piece& pc : cols[c]) e2 += bfp_err2(pc.src, pc.m, kBfpBits[f]);
+            ok = (e2 <= thr[c]);
+        }
+        if (ok) return kFirstBfp + f;
+    }
+    return rate <= 23 ? F32 : F64;
+}
+
+// write NB-bit value q as field j of the 16-byte row w[0..3]
+void bfp_put(uint32_t* w, int j, int nb, int32_t q) {
+    const uint32_t u = uint32_t(q) & ((1u << nb) - 1u);
+    const int bit = j * nb, word = bit >> 5, off = bit & 31;
+    w[word] |= u << off;
+    if (off + nb > 32) w[word + 1] |= u >> (32 - off);
+}
 
 int eps_to_rate(double eps) {
     const double r = std::ceil(-std::log2(eps));
@@ -134,8 +218,9 @@ struct Seg {              // a block's contribution to one panel
 int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t ncols, const double* arena,
                  int64_t arena_len, const hmvm_plan_opts& opts, HostLayout& L) {
     if (nb < 0 || nrows < 0 || ncols < 0 || (nb > 0 && (!blocks || !arena))) return HMVM_ERR_INVALID_ARG;
-    if (opts.precision < HMVM_PREC_FP64 || opts.precision > HMVM_PREC_VALR) return HMVM_ERR_INVALID_ARG;
-    if (opts.precision == HMVM_PREC_VALR && !(opts.eps > 0)) return HMVM_ERR_INVALID_ARG;
+    if (opts.precision < HMVM_PREC_FP64 || opts.precision > HMVM_PREC_BFP) return HMVM_ERR_INVALID_ARG;
+    const bool adaptive = (opts.precision == HMVM_PREC_VALR || opts.precision == HMVM_PREC_BFP);
+    if (adaptive && !(opts.eps > 0)) return HMVM_ERR_INVALID_ARG;
     if (opts.force_format < 0 || opts.force_format > NFMT) return HMVM_ERR_INVALID_ARG;
     if (nrows > INT32_MAX || ncols > INT32_MAX) return HMVM_ERR_UNSUPPORTED;
     L = HostLayout();
@@ -145,8 +230,9 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     L.eps = opts.eps;
     // dense leaves under VALR: as requested, or (dense_prec == VALR) by the
     // same rate rule as the columns, restricted to FP32 / FP64: FP32 while
-    // eps needs at most its 23 mantissa bits
-    if (opts.precision == HMVM_PREC_VALR && opts.dense_prec != HMVM_PREC_FP64 &&
+    // eps needs at most its 23 mantissa bits.  Under BFP with dense_prec ==
+    // VALR every dense column gets the block floating point rule at rate(eps).
+    if (adaptive && opts.dense_prec != HMVM_PREC_FP64 &&
         opts.dense_prec != HMVM_PREC_FP32 && opts.dense_prec != HMVM_PREC_VALR)
         return HMVM_ERR_INVALID_ARG;
     L.dense_fmt = (opts.precision == HMVM_PREC_FP64) ? F64
@@ -154,6 +240,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 : (opts.dense_prec == HMVM_PREC_FP64) ? F64
                 : (opts.dense_prec == HMVM_PREC_FP32) ? F32
                 : (eps_to_rate(opts.eps) <= 23 ? F32 : F64);
+    const bool dense_bfp = (opts.precision == HMVM_PREC_BFP && opts.dense_prec == HMVM_PREC_VALR);
     L.blocks.assign(blocks, blocks + nb);
 
     // ---- validate leaves (cf. HLR_ASSERT in apply_add, check_data()) ----
@@ -258,7 +345,52 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             }
             if (opts.force_format > 0)
                 for (int64_t l = 0; l < b.rank; ++l) L.tfmt[e0 + l] = int8_t(opts.force_format - 1);
-            for (int64_t l = 0; l < b.rank; ++l) L.cols_by_format[L.tfmt[e0 + l]]++;
+        }
+    }
+    // block floating point: U and V column l of a block share one width, the
+    // narrowest whose panel columns meet the 2-norm bound for rate(u_l)
+    auto upieces = [&](int64_t i, int64_t l) {
+        const hmvm_block& b = blocks[i];
+        std::vector<Piece> v;
+        for (int64_t p = bp0[i]; p < bp1[i]; ++p)
+            v.push_back({arena + b.off_a + l * int64_t(b.nrows) + (rstart[p] - b.row_ofs), rpan[p].second});
+        return v;
+    };
+    if (opts.precision == HMVM_PREC_BFP && opts.force_format == 0) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < nb; ++i) {
+            const hmvm_block& b = blocks[i];
+            if (b.kind == HMVM_BLOCK_DENSE || b.rank == 0 || b.off_s < 0) continue;
+            std::vector<double> tol(b.rank);
+            valr_tolerances(b.rank, arena + b.off_s, opts.eps, tol.data());
+            for (int64_t l = 0; l < b.rank; ++l) {
+                std::vector<std::vector<Piece>> cols(2);
+                cols[0] = upieces(i, l);
+                for (int64_t q = bq0[i]; q < bq1[i]; ++q)
+                    cols[1].push_back({arena + b.off_b + l * int64_t(b.ncols) + (cstart[q] - b.col_ofs),
+                                       cpan[q].second});
+                L.tfmt[L.blk_t0[i] + l] = int8_t(bfp_choose(cols, eps_to_rate(tol[l])));
+            }
+        }
+    }
+    for (int64_t e = 0; e < ne; ++e) L.cols_by_format[L.tfmt[e]]++;
+    // dense columns: one format per column of each dense leaf
+    L.blk_d0.assign(nb, -1);
+    {
+        int64_t nd = 0;
+        for (int64_t i = 0; i < nb; ++i)
+            if (blocks[i].kind == HMVM_BLOCK_DENSE) { L.blk_d0[i] = nd; nd += blocks[i].ncols; }
+        L.dfmt.assign(nd, int8_t(L.dense_fmt));
+        if (dense_bfp) {
+            const int rate = eps_to_rate(opts.eps);
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int64_t i = 0; i < nb; ++i) {
+                if (blocks[i].kind != HMVM_BLOCK_DENSE) continue;
+                for (int64_t j = 0; j < blocks[i].ncols; ++j) {
+                    std::vector<std::vector<Piece>> cols(1, upieces(i, j));
+                    L.dfmt[L.blk_d0[i] + j] = int8_t(bfp_choose(cols, rate));
+                }
+            }
         }
     }
 
@@ -364,20 +496,20 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     }
 
     // ---- row panels: column counts per format, chunks ----
-    const int dfmt = L.dense_fmt;
     L.rp.resize(NR);
     std::vector<std::array<int64_t, NFMT>> rcols(NR);
     for (int64_t p = 0; p < NR; ++p) {
-        std::array<int64_t, NFMT> c = {0, 0, 0, 0};
+        std::array<int64_t, NFMT> c{};
         for (const Seg& s : rseg[p]) {
             const hmvm_block& b = blocks[s.b];
-            if (b.kind == HMVM_BLOCK_DENSE) c[dfmt] += b.ncols;
+            if (b.kind == HMVM_BLOCK_DENSE)
+                for (int64_t j = 0; j < b.ncols; ++j) c[L.dfmt[L.blk_d0[s.b] + j]]++;
             else for (int64_t l = 0; l < b.rank; ++l) c[L.tfmt[L.blk_t0[s.b] + l]]++;
         }
         rcols[p] = c;
     }
     // chunk table and offsets
-    int64_t arena_pos[NFMT] = {0, 0, 0, 0};
+    int64_t arena_pos[NFMT] = {};
     int64_t colpos = 0;
     std::vector<std::array<int64_t, NFMT>> rcol0(NR);        // first per-column index of format f in panel p
     for (int64_t p = 0; p < NR; ++p) {
@@ -421,13 +553,14 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     L.cp.resize(NC);
     std::vector<std::array<int64_t, NFMT>> ccols(NC);
     for (int64_t q = 0; q < NC; ++q) {
-        std::array<int64_t, NFMT> c = {0, 0, 0, 0};
+        std::array<int64_t, NFMT> c{};
         for (const Seg& s : cseg[q])
             for (int64_t l = 0; l < blocks[s.b].rank; ++l) c[L.tfmt[L.blk_t0[s.b] + l]]++;
         ccols[q] = c;
     }
     int64_t vpos = 0;
-    int64_t varena_pos[NFMT] = {arena_pos[0], arena_pos[1], arena_pos[2], arena_pos[3]};
+    int64_t varena_pos[NFMT];
+    for (int f = 0; f < NFMT; ++f) varena_pos[f] = arena_pos[f];
     for (int64_t q = 0; q < NC; ++q) {
         ColPanel& C = L.cp[q];
         C.col0 = cpan[q].first;
@@ -444,7 +577,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             // that the forward packs merge with remainders of other panels;
             // each segment is a contiguous m x (ng * W) row-major block that
             // starts on a 128-byte boundary
-            const int64_t align = 128 / fmt_bytes(f);
+            const int64_t align = 8 * int64_t(W);          // values in 128 bytes
             for (int64_t g = 0; g < ng; g += kItemGroups) {
                 const int64_t cnt = std::min<int64_t>(kItemGroups, ng - g);
                 varena_pos[f] = round_up(varena_pos[f], align);
@@ -488,10 +621,12 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     if (nrcols > INT32_MAX || nvcols > INT32_MAX) return HMVM_ERR_UNSUPPORTED;
 
     // ---- allocate arenas and tables ----
+    for (int f = 0; f < NFMT; ++f) L.nvals[f] = varena_pos[f];
     L.a64.assign(varena_pos[F64], 0.0);
     L.a32.assign(varena_pos[F32], 0.0f);
     L.a16.assign(varena_pos[BF16], 0);
     L.a8.assign(varena_pos[FP8], 0);
+    for (int f = kFirstBfp; f < NFMT; ++f) L.abfp[f - kFirstBfp].assign(varena_pos[f] / fmt_width(f) * 4, 0u);
     L.colsrc.assign(nrcols, kPadCol);
     L.rdest.assign(nrcols, -1);
     L.vdest.assign(nvcols, -1);
@@ -502,26 +637,39 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         L.vscale.assign(nvcols, 1.0f);
     }
 
-    // store one column of m values (source stride 1) in format f at group layout
-    auto store_col = [&](int f, int64_t base, int64_t m, int64_t W, int64_t c, const double* src,
-                         float* scale_out) {
-        // column c of a format group block starting at element `base` with
-        // groups of W columns, each m x W row-major
-        const int64_t g = c / W, w = c % W;
-        const int64_t o = base + g * m * W + w;
+    // store m values src[0..m) of one panel column in format f at value
+    // offsets o, o + ld, ..., o + (m-1) ld; returns the column's scale
+    auto store_vals = [&](int f, int64_t o, int64_t ld, int64_t m, const double* src) -> float {
         switch (f) {
-        case F64: for (int64_t i = 0; i < m; ++i) L.a64[o + i * W] = src[i]; break;
-        case F32: for (int64_t i = 0; i < m; ++i) L.a32[o + i * W] = float(src[i]); break;
-        case BF16: for (int64_t i = 0; i < m; ++i) L.a16[o + i * W] = encode_bf16(float(src[i])); break;
+        case F64: for (int64_t i = 0; i < m; ++i) L.a64[o + i * ld] = src[i]; return 1.0f;
+        case F32: for (int64_t i = 0; i < m; ++i) L.a32[o + i * ld] = float(src[i]); return 1.0f;
+        case BF16: for (int64_t i = 0; i < m; ++i) L.a16[o + i * ld] = encode_bf16(float(src[i])); return 1.0f;
         case FP8: {
             double mx = 0;
             for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
             const float s = fp8_scale(mx);
-            for (int64_t i = 0; i < m; ++i) L.a8[o + i * W] = encode_e4m3(float(src[i] / double(s)));
-            if (scale_out) *scale_out = s;
-            break;
+            for (int64_t i = 0; i < m; ++i) L.a8[o + i * ld] = encode_e4m3(float(src[i] / double(s)));
+            return s;
+        }
+        default: {
+            const int nb = fmt_bits(f), W = fmt_width(f);
+            double mx = 0;
+            for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
+            const int k = bfp_exp(mx, nb);
+            uint32_t* words = L.abfp[f - kFirstBfp].data();
+            for (int64_t i = 0; i < m; ++i) {
+                const int64_t v = o + i * ld;
+                bfp_put(words + (v / W) * 4, int(v % W), nb, bfp_quant(src[i], k, nb));
+            }
+            return std::ldexp(1.0f, k);
         }
         }
+    };
+    // column c of a format group block starting at value offset `base` with
+    // groups of W columns, each m x W row-major
+    auto store_col = [&](int f, int64_t base, int64_t m, int64_t W, int64_t c, const double* src) {
+        const int64_t g = c / W, w = c % W;
+        return store_vals(f, base + g * m * W + w, W, m, src);
     };
 
     // ---- fill row panels (parallel over panels) ----
@@ -529,9 +677,10 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     for (int64_t p = 0; p < NR; ++p) {
         const RowPanel& P = L.rp[p];
         const int64_t m = P.m;
-        int64_t cnt[NFMT] = {0, 0, 0, 0};
-        // element base of format f block of this panel = data_off of its first chunk
-        int64_t fbase[NFMT] = {-1, -1, -1, -1};
+        int64_t cnt[NFMT] = {};
+        // value base of format f block of this panel = data_off of its first chunk
+        int64_t fbase[NFMT];
+        for (int f = 0; f < NFMT; ++f) fbase[f] = -1;
         for (int64_t c = P.chunk0; c < P.chunk0 + P.nchunks; ++c)
             if (fbase[L.rchunks[c].fmt] < 0) fbase[L.rchunks[c].fmt] = L.rchunks[c].data_off;
         std::vector<double> tmp(m);
@@ -539,12 +688,11 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             const hmvm_block& b = blocks[s.b];
             const int64_t r0 = P.row0 - b.row_ofs;     // first row of the panel inside the block
             if (b.kind == HMVM_BLOCK_DENSE) {
-                const int f = dfmt, W = fmt_width(f);
                 for (int64_t j = 0; j < b.ncols; ++j) {
+                    const int f = L.dfmt[L.blk_d0[s.b] + j], W = fmt_width(f);
                     const int64_t c = cnt[f]++;
                     const double* src = arena + b.off_a + j * int64_t(b.nrows) + r0;
-                    float sc = 1.0f;
-                    store_col(f, fbase[f], m, W, c, src, &sc);
+                    const float sc = store_col(f, fbase[f], m, W, c, src);
                     const int64_t gc = rcol0[p][f] + c;
                     L.colsrc[gc] = int32_t(b.col_ofs + j);
                     if (need_scale) L.zscale[gc] = sc;
@@ -556,8 +704,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                     const int f = L.tfmt[e0 + l], W = fmt_width(f);
                     const int64_t c = cnt[f]++;
                     const double* src = arena + b.off_a + l * int64_t(b.nrows) + r0;
-                    float sc = 1.0f;
-                    store_col(f, fbase[f], m, W, c, src, &sc);
+                    const float sc = store_col(f, fbase[f], m, W, c, src);
                     const int64_t gc = rcol0[p][f] + c;
                     L.colsrc[gc] = ~int32_t(e0 + l);
                     if (need_scale) L.zscale[gc] = sc;
@@ -572,7 +719,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     for (int64_t q = 0; q < NC; ++q) {
         const ColPanel& C = L.cp[q];
         const int64_t m = C.m;
-        int64_t cnt[NFMT] = {0, 0, 0, 0};
+        int64_t cnt[NFMT] = {};
         for (const Seg& s : cseg[q]) {
             const hmvm_block& b = blocks[s.b];
             const int64_t r0 = C.col0 - b.col_ofs;
@@ -590,19 +737,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 const int64_t ld = int64_t(it->ng) * W;
                 const int64_t o = it->data_off + (c - int64_t(it->g0) * W);
                 const double* src = arena + b.off_b + l * int64_t(b.ncols) + r0;
-                float sc = 1.0f;
-                switch (f) {
-                case F64: for (int64_t i = 0; i < m; ++i) L.a64[o + i * ld] = src[i]; break;
-                case F32: for (int64_t i = 0; i < m; ++i) L.a32[o + i * ld] = float(src[i]); break;
-                case BF16: for (int64_t i = 0; i < m; ++i) L.a16[o + i * ld] = encode_bf16(float(src[i])); break;
-                case FP8: {
-                    double mx = 0;
-                    for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
-                    sc = fp8_scale(mx);
-                    for (int64_t i = 0; i < m; ++i) L.a8[o + i * ld] = encode_e4m3(float(src[i] / double(sc)));
-                    break;
-                }
-                }
+                const float sc = store_vals(f, o, ld, m, src);
                 const int64_t vc = C.vcol0[f] + c;
                 L.vdest[vc] = L.pt_off[e0 + l] + s.pi * int32_t(b.rank);
                 L.vsrc[vc] = int32_t(e0 + l);
@@ -630,38 +765,40 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     }
 
     // ---- statistics ----
-    L.bytes_by_format[F64] = int64_t(L.a64.size()) * 8;
-    L.bytes_by_format[F32] = int64_t(L.a32.size()) * 4;
-    L.bytes_by_format[BF16] = int64_t(L.a16.size()) * 2;
-    L.bytes_by_format[FP8] = int64_t(L.a8.size());
+    for (int f = 0; f < NFMT; ++f) L.bytes_by_format[f] = fmt_store_bytes(f, L.nvals[f]);
+    // bytes per stored value (16 bytes per row of W values)
+    auto vbytes = [](int f) { return 16.0 / double(fmt_width(f)); };
     double fl = 0, ds = 0, br = 0, bv = 0;
     for (int64_t i = 0; i < nb; ++i) {
         const hmvm_block& b = blocks[i];
         const double m = b.nrows, n = b.ncols;
         if (b.kind == HMVM_BLOCK_DENSE) {
             fl += 2 * m * n + 2 * m;
-            ds += fmt_bytes(dfmt) * m * n;
-            br += fmt_bytes(dfmt) * m * n;
+            for (int64_t j = 0; j < b.ncols; ++j) {
+                const double by = vbytes(L.dfmt[L.blk_d0[i] + j]) * m;
+                ds += by;
+                br += by;
+            }
         } else {
             const double k = b.rank;
             fl += 2 * k * (m + n) + 2 * (m + n);
             if (b.kind == HMVM_BLOCK_LOWRANK_SV) ds += 8.0 * k;   // S (matrix/lrsvmatrix.hh:364-370)
             for (int64_t l = 0; l < b.rank; ++l) {
                 const int f = L.tfmt[L.blk_t0[i] + l];
-                ds += fmt_bytes(f) * (m + n);
-                br += fmt_bytes(f) * m;
-                bv += fmt_bytes(f) * n;
+                ds += vbytes(f) * (m + n);
+                br += vbytes(f) * m;
+                bv += vbytes(f) * n;
             }
         }
     }
-    // the scale of an FP8 panel column counts as data (4 bytes per column of
-    // an FP8 chunk or segment, which the kernels read; other columns' table
-    // entries are never read)
+    // the scale of an FP8 or block floating point panel column counts as
+    // data (4 bytes per column of such a chunk or segment, which the kernels
+    // read; other columns' table entries are never read)
     int64_t s8r = 0, s8v = 0;
     for (const RChunk& c : L.rchunks)
-        if (c.fmt == FP8) s8r += int64_t(c.ngroups) * fmt_width(FP8);
+        if (fmt_scaled(c.fmt)) s8r += int64_t(c.ngroups) * fmt_width(c.fmt);
     for (const VItem& v : L.vitems)
-        if (v.fmt == FP8) s8v += int64_t(v.ng) * fmt_width(FP8);
+        if (fmt_scaled(v.fmt)) s8v += int64_t(v.ng) * fmt_width(v.fmt);
     L.scale_bytes = 4 * (s8r + s8v);
     br += 4.0 * double(s8r);
     bv += 4.0 * double(s8v);
@@ -677,17 +814,34 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
 // unpack (host copies of the arenas)
 // ---------------------------------------------------------------------------
 
-static double load_elem(int f, const void* const arenas[4], int64_t idx, bool as_f64) {
+template <int NB>
+static int32_t bfp_get(const uint32_t* words, int64_t v) {
+    constexpr int W = 128 / NB;
+    const uint32_t* r = words + (v / W) * 4;
+    const uint32_t w[4] = {r[0], r[1], r[2], r[3]};
+    return bfp_field<NB>(w, int(v % W));
+}
+
+// stored value idx of format f (without its panel-column scale)
+static double load_elem(int f, const void* const arenas[NFMT], int64_t idx, bool as_f64) {
     if (as_f64) return static_cast<const double*>(arenas[f])[idx];
+    const uint32_t* w = static_cast<const uint32_t*>(arenas[f]);
     switch (f) {
-    case F64: return static_cast<const double*>(arenas[0])[idx];
-    case F32: return static_cast<const float*>(arenas[1])[idx];
-    case BF16: return decode_bf16(static_cast<const uint16_t*>(arenas[2])[idx]);
-    default: return decode_e4m3(static_cast<const uint8_t*>(arenas[3])[idx]);
+    case F64: return static_cast<const double*>(arenas[F64])[idx];
+    case F32: return static_cast<const float*>(arenas[F32])[idx];
+    case BF16: return decode_bf16(static_cast<const uint16_t*>(arenas[BF16])[idx]);
+    case FP8: return decode_e4m3(static_cast<const uint8_t*>(arenas[FP8])[idx]);
+    case B8: return bfp_get<8>(w, idx);
+    case B10: return bfp_get<10>(w, idx);
+    case B12: return bfp_get<12>(w, idx);
+    case B16: return bfp_get<16>(w, idx);
+    case B21: return bfp_get<21>(w, idx);
+    default: return bfp_get<25>(w, idx);
     }
 }
 
-int unpack_block(const HostLayout& L, int64_t bi, const void* const arenas[4], bool as_f64, double* a, double* bb) {
+int unpack_block(const HostLayout& L, int64_t bi, const void* const arenas[NFMT], bool as_f64, double* a,
+                 double* bb) {
     if (bi < 0 || bi >= int64_t(L.blocks.size()) || !a) return HMVM_ERR_INVALID_ARG;
     const hmvm_block& b = L.blocks[bi];
     const bool dense = (b.kind == HMVM_BLOCK_DENSE);

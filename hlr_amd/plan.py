@@ -13,9 +13,10 @@ import numpy as np
 
 from . import _native as N
 
-PRECISIONS = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "valr": N.PREC_VALR}
-FORMAT_NAMES = ("fp64", "fp32", "bf16", "fp8_e4m3")
-FORMAT_BYTES = (8, 4, 2, 1)
+PRECISIONS = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "valr": N.PREC_VALR, "bfp": N.PREC_BFP}
+FORMAT_NAMES = ("fp64", "fp32", "bf16", "fp8_e4m3", "bfp8", "bfp10", "bfp12", "bfp16", "bfp21", "bfp25")
+# bytes per stored value (16 bytes hold 128 // bits values)
+FORMAT_BYTES = tuple(16.0 / (128 // b) for b in N.FMT_BITS)
 
 
 def _stream_handle(stream):
@@ -49,8 +50,9 @@ class Plan:
         arena = np.ascontiguousarray(arena, dtype=np.float64)
         opts = N.HmvmPlanOpts()
         opts.precision = PRECISIONS[precision]
-        # dense leaves under "valr": "fp64", "fp32", or "auto" (FP32 while eps
-        # needs at most 23 mantissa bits, else FP64)
+        # dense leaves under "valr" / "bfp": "fp64", "fp32", or "auto" (valr:
+        # FP32 while eps needs at most 23 mantissa bits, else FP64; bfp: the
+        # block floating point column rule at rate(eps))
         opts.dense_prec = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "auto": N.PREC_VALR}[dense_prec]
         opts.eps = float(eps)
         opts.check_finite = 1 if check_finite else 0
@@ -154,10 +156,14 @@ class Plan:
         N.check(N.hmvm().hmvm_plan_arena_elems(self._p, fmt, C.byref(e)), "hmvm_plan_arena_elems")
         return int(e.value)
 
+    def arena_bytes(self, fmt: int) -> int:
+        return int(self.info()["bytes_by_format"][fmt])
+
     def arena_numpy(self, fmt: int) -> np.ndarray:
-        """Host copy of the stored arena of format fmt (host-only or device plan)."""
-        dt = (np.float64, np.float32, np.uint16, np.uint8)[fmt]
-        out = np.zeros(self.arena_elems(fmt), dtype=dt)
+        """Host copy of the stored arena of format fmt (host-only or device
+        plan); block floating point arenas come as their raw bytes."""
+        dt = (np.float64, np.float32, np.uint16, np.uint8)[fmt] if fmt < N.FMT_B8 else np.uint8
+        out = np.zeros(self.arena_bytes(fmt) // np.dtype(dt).itemsize, dtype=dt)
         if out.size == 0:
             return out
         if self.device < 0:
@@ -181,8 +187,9 @@ class Plan:
         dense = int(blk["kind"]) == N.BLOCK_DENSE
         a = np.zeros((m * n) if dense else (m * k))
         bb = None if dense else np.zeros(n * k)
-        ptrs = (C.c_void_p * 4)(*[C.c_void_p(arr.ctypes.data) if arr is not None and arr.size else None
-                                  for arr in arenas])
+        arenas = list(arenas) + [None] * (N.NFMT - len(arenas))
+        ptrs = (C.c_void_p * N.NFMT)(*[C.c_void_p(arr.ctypes.data) if arr is not None and arr.size else None
+                                       for arr in arenas])
         N.check(N.hmvm().hmvm_plan_unpack_block(self._p, b, ptrs, 1 if as_f64 else 0, C.c_void_p(a.ctypes.data),
                                                 C.c_void_p(bb.ctypes.data) if bb is not None else None),
                 "hmvm_plan_unpack_block")
@@ -194,10 +201,10 @@ class Plan:
         """Adjoint of y = A x: gx += A^T ybar, and per-format FP64 gradient
         arenas (dL/d stored factor value, in the plan's layout)."""
         import torch
-        garena = [None] * 4
-        ptrs = (C.c_void_p * 4)()
+        garena = [None] * N.NFMT
+        ptrs = (C.c_void_p * N.NFMT)()
         if factors:
-            for f in range(4):
+            for f in range(N.NFMT):
                 e = self.arena_elems(f)
                 if e:
                     garena[f] = torch.zeros(e, dtype=torch.float64, device=x.device)

@@ -63,9 +63,15 @@ enum {
 enum {
     HMVM_PREC_FP64 = 0,   /* plain path: every factor and accumulation in FP64          */
     HMVM_PREC_FP32 = 1,   /* every factor FP32, FP32 accumulation inside a panel        */
-    HMVM_PREC_VALR = 2    /* per-column FP64/FP32/BF16/FP8-E4M3 chosen from singular
+    HMVM_PREC_VALR = 2,   /* per-column FP64/FP32/BF16/FP8-E4M3 chosen from singular
                              values against eps (compress/valr.hh:402-512,
                              compress/afl.hh:81-82,854-856); dense leaves FP32          */
+    HMVM_PREC_BFP  = 3    /* per-column block floating point (compress/bfp.hh:765-860
+                             with VALR tolerances): the narrowest of the B8..B25
+                             formats whose stored column, U and V alike, is within
+                             2^-(rate+1) of the FP64 column in the 2-norm, rate from
+                             the column's VALR tolerance; FP32 / FP64 when none is.
+                             Dense leaves by the same rule at rate(eps) per column    */
 };
 
 /* vector element type at the boundary */
@@ -92,20 +98,32 @@ typedef struct hmvm_block {
     int64_t off_s;    /* singular values (k), or -1 when unknown   */
 } hmvm_block;
 
-/* storage formats of factor columns (index of the plan's four arenas) */
+/* storage formats of factor columns (index of the plan's arenas) */
 enum {
     HMVM_FMT_F64  = 0,
     HMVM_FMT_F32  = 1,
     HMVM_FMT_BF16 = 2,
-    HMVM_FMT_FP8  = 3     /* OCP FP8 E4M3 with a power-of-two scale per panel column */
+    HMVM_FMT_FP8  = 3,    /* OCP FP8 E4M3 with a power-of-two scale per panel column */
+    /* block floating point: NB-bit two's-complement integers q packed
+       floor(128 / NB) to a 16-byte row (value j in bits [j NB, j NB + NB) of
+       the row as a little-endian 128-bit integer), value q * 2^k with one
+       power-of-two step 2^k per panel column                                 */
+    HMVM_FMT_B8   = 4,    /* NB =  8, 16 values per row */
+    HMVM_FMT_B10  = 5,    /* NB = 10, 12 values per row */
+    HMVM_FMT_B12  = 6,    /* NB = 12, 10 values per row */
+    HMVM_FMT_B16  = 7,    /* NB = 16,  8 values per row */
+    HMVM_FMT_B21  = 8,    /* NB = 21,  6 values per row */
+    HMVM_FMT_B25  = 9     /* NB = 25,  5 values per row */
 };
+#define HMVM_NFMT 10
 
 typedef struct hmvm_plan_opts {
     int32_t precision;    /* HMVM_PREC_*                                          */
-    int32_t dense_prec;   /* dense leaves under HMVM_PREC_VALR: HMVM_PREC_FP64,
-                             HMVM_PREC_FP32, or HMVM_PREC_VALR = by eps (FP32 while
-                             ceil(-log2 eps) <= 23, else FP64)                   */
-    double  eps;          /* compression tolerance for HMVM_PREC_VALR             */
+    int32_t dense_prec;   /* dense leaves under HMVM_PREC_VALR / _BFP: HMVM_PREC_FP64,
+                             HMVM_PREC_FP32, or HMVM_PREC_VALR = by eps (VALR: FP32
+                             while ceil(-log2 eps) <= 23, else FP64; BFP: the
+                             column rule of HMVM_PREC_BFP at rate(eps))          */
+    double  eps;          /* compression tolerance for HMVM_PREC_VALR / _BFP      */
     int32_t check_finite; /* reject NaN/Inf factors at creation (default 1)       */
     int32_t force_format; /* 0: formats from `precision`; 1 + HMVM_FMT_*: store
                              every low-rank column in that format (experiments
@@ -117,9 +135,10 @@ typedef struct hmvm_plan_info {
     int64_t nblocks, ndense, nlowrank;
     int64_t nrow_panels, ncol_panels;
     int64_t factor_bytes;        /* device bytes of all stored factors           */
-    int64_t bytes_by_format[4];  /* FP64, FP32, BF16, FP8 factor bytes           */
-    int64_t cols_by_format[4];   /* low-rank columns stored in each format       */
-    int64_t scale_bytes;         /* FP8 column scales (4 B per FP8 panel column) */
+    int64_t bytes_by_format[HMVM_NFMT]; /* factor bytes per HMVM_FMT_*       */
+    int64_t cols_by_format[HMVM_NFMT];  /* low-rank columns per HMVM_FMT_*   */
+    int64_t scale_bytes;         /* panel-column scales of the FP8 and block
+                                    floating point formats (4 B each)            */
     double  flops;               /* mul_vec_flops for nrhs = 1                   */
     double  datasize;            /* mul_vec_datasize restated for this storage   */
     int64_t index_bytes;         /* per-column source/destination index tables   */
@@ -209,9 +228,10 @@ int hmvm_tlr_apply(const hmvm_plan* plan, double alpha, int op,
  * product follows arith/blas.hh:910-948.
  */
 int hmvm_grad(const hmvm_plan* plan, const void* x, const void* ybar,
-              void* gx, void* grad_arena[4], int vec_dtype, void* stream);
+              void* gx, void* grad_arena[HMVM_NFMT], int vec_dtype, void* stream);
 
-/* number of elements of the device arena of storage format f (0..3) */
+/* number of stored values of the arena of storage format f (0..HMVM_NFMT-1);
+   its bytes are bytes_by_format[f] (16 bytes per floor(128 / bits) values) */
 int hmvm_plan_arena_elems(const hmvm_plan* plan, int fmt, int64_t* elems);
 
 /* copy a device arena out of / into the plan (device to device) */
@@ -227,7 +247,7 @@ int hmvm_plan_set_arena(hmvm_plan* plan, int fmt, const void* src, void* stream)
  * hmvm_grad, taken as they are.
  */
 int hmvm_plan_unpack_block(const hmvm_plan* plan, int64_t block,
-                           const void* const host_arenas[4], int as_f64,
+                           const void* const host_arenas[HMVM_NFMT], int as_f64,
                            double* a, double* b);
 
 /* ------------------------------------------------------------------------ */

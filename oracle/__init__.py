@@ -13,6 +13,8 @@ header and exercised by tests/test_oracle.py):
                                            singular values)
   round_bf16, round_e4m3                   pinned (torch dtype conversions and
                                            the OCP E4M3 code table)
+  bfp_exp, bfp_round, bfp_format           pinned (numpy rint, brute-force
+                                           minimality, representable columns)
 """
 from __future__ import annotations
 
@@ -61,6 +63,12 @@ def lib() -> C.CDLL:
         L.oracle_decode_e4m3.restype = d
         L.oracle_round_e4m3.argtypes = [d]
         L.oracle_round_e4m3.restype = d
+        L.oracle_bfp_exp.argtypes = [d, i32]
+        L.oracle_bfp_exp.restype = i32
+        L.oracle_bfp_round.argtypes = [d, i32, i32]
+        L.oracle_bfp_round.restype = d
+        L.oracle_bfp_format.argtypes = [i32, vp, vp, vp, i32]
+        L.oracle_bfp_format.restype = i32
         _LIB = L
     return _LIB
 
@@ -145,3 +153,48 @@ def decode_e4m3(c):
 
 def round_e4m3(v):
     return lib().oracle_round_e4m3(float(v))
+
+
+def bfp_exp(maxabs, nb):
+    return lib().oracle_bfp_exp(float(maxabs), int(nb))
+
+
+def bfp_round(v, k, nb):
+    return lib().oracle_bfp_round(float(v), int(k), int(nb))
+
+
+def _pieces(n, piece):
+    if np.isscalar(piece):
+        return [piece] * (n // piece) + ([n % piece] if n % piece else [])
+    return list(piece)
+
+
+def bfp_round_column(col, nb, piece=64):
+    """Stored values of a column in panel columns of `piece` rows (or the
+    given list of piece lengths), each with its own step."""
+    col = np.asarray(col, dtype=np.float64)
+    out = np.empty_like(col)
+    r0 = 0
+    for ln in _pieces(len(col), piece):
+        seg = col[r0:r0 + ln]
+        k = bfp_exp(np.abs(seg).max() if ln else 0.0, nb)
+        out[r0:r0 + ln] = [bfp_round(v, k, nb) for v in seg]
+        r0 += ln
+    return out
+
+
+def bfp_format(cols, rate, piece=64):
+    """Format chosen for the columns `cols` (list of 1-D arrays) sharing one
+    width, each cut into panel columns of `piece` rows (or one list of piece
+    lengths per column)."""
+    vals, npieces, plen = [], [], []
+    for c, col in enumerate(cols):
+        col = np.asarray(col, dtype=np.float64)
+        lens = _pieces(len(col), piece if np.isscalar(piece) else piece[c])
+        vals.append(col)
+        npieces.append(len(lens))
+        plen.extend(lens)
+    v = np.ascontiguousarray(np.concatenate(vals) if vals else np.zeros(0))
+    npc = np.ascontiguousarray(npieces, dtype=np.int64)
+    pl = np.ascontiguousarray(plen, dtype=np.int64)
+    return lib().oracle_bfp_format(len(cols), _p(v), _p(npc), _p(pl), int(rate))

@@ -27,12 +27,20 @@
  *   oracle_valr_tolerances  valr::get_tolerances (compress/valr.hh:402-512)
  *   oracle_eps_to_rate    afl::eps_to_rate (compress/afl.hh:81-82)
  *   oracle_round_*        round-to-nearest-even into BF16 / FP8-E4M3 (OCP)
+ *   oracle_bfp_*          block floating point columns: bfp::compress_lr
+ *                         (compress/bfp.hh:765-860) with a power-of-two step
+ *                         per panel column and value widths of 8, 10, 12, 16,
+ *                         21 or 25 bits; the width of a column is the
+ *                         narrowest whose stored column is within
+ *                         2^-(rate+1) of the column in the 2-norm
  *   oracle_dense_expand   sum of the leaves placed at their offsets
  *
  * Pins (tests/): dense expansion against direct kernel evaluation and
  * numpy products; linearity, accumulate, alpha = 0, <Ax,w> = <x,A^T w>,
  * TLR == H on one-level trees, brute force on hand-built trees; rounding
- * against torch's float8_e4m3fn / bfloat16 conversions and the OCP table.
+ * against torch's float8_e4m3fn / bfloat16 conversions and the OCP table;
+ * block floating point against numpy's rint, brute-force minimality of the
+ * step and of the chosen width, and exactly representable columns.
  */
 #include <math.h>
 #include <omp.h>
@@ -409,3 +417,76 @@ double oracle_decode_e4m3(uint8_t c) {
 }
 
 double oracle_round_e4m3(double v) { return oracle_decode_e4m3(oracle_encode_e4m3(v)); }
+
+/*
+ * Block floating point.  A panel column of NB-bit values stores q_i with
+ * v_i ~ q_i 2^k, |q_i| <= 2^(NB-1) - 1, one step 2^k per panel column.
+ */
+
+/* smallest k with maxabs <= (2^(nb-1) - 1) 2^k, clamped to [-126, 127]
+   (the step is kept as a normal FP32 number); 0 for maxabs == 0 */
+int oracle_bfp_exp(double maxabs, int nb) {
+    if (!(maxabs > 0) || isinf(maxabs)) return 0;
+    const double qmax = ldexp(1.0, nb - 1) - 1.0;
+    int e;
+    (void)frexp(maxabs, &e);                     /* 2^(e-1) <= maxabs < 2^e */
+    int k = e - nb - 1;                          /* qmax 2^k < 2^(e-2): too small */
+    while (maxabs > ldexp(qmax, k)) ++k;
+    if (k < -126) k = -126;
+    if (k > 127) k = 127;
+    return k;
+}
+
+/* nearest multiple q 2^k of v (ties to even q), |q| <= 2^(nb-1) - 1 */
+double oracle_bfp_round(double v, int k, int nb) {
+    const double qmax = ldexp(1.0, nb - 1) - 1.0;
+    const double r = ldexp(v, -k);
+    const double fl = floor(r), diff = r - fl;
+    double q;
+    if (diff > 0.5) q = fl + 1;
+    else if (diff < 0.5) q = fl;
+    else q = (fmod(fl, 2.0) == 0.0) ? fl : fl + 1;
+    if (q > qmax) q = qmax;
+    if (q < -qmax) q = -qmax;
+    return ldexp(q, k);
+}
+
+/* value widths of the formats HMVM_FMT_B8 .. HMVM_FMT_B25 */
+static const int oracle_bfp_bits[6] = {8, 10, 12, 16, 21, 25};
+
+/*
+ * Storage format of one column group: ncols columns, values concatenated in
+ * `vals`; column c consists of npieces[c] panel columns whose lengths follow
+ * one another in `plen`.  Returns the first of HMVM_FMT_B8 .. HMVM_FMT_B25
+ * for which every column's stored pieces are within 2^-(rate+1) of the
+ * column in the 2-norm, else HMVM_FMT_F32 for rate <= 23, else HMVM_FMT_F64.
+ */
+int oracle_bfp_format(int ncols, const double* vals, const int64_t* npieces, const int64_t* plen, int rate) {
+    for (int f = 0; f < 6; ++f) {
+        const int nb = oracle_bfp_bits[f];
+        int ok = 1;
+        const double* v = vals;
+        const int64_t* pl = plen;
+        for (int c = 0; c < ncols; ++c) {
+            const double* v0 = v;
+            const int64_t* pl0 = pl;
+            double n2 = 0.0, e2 = 0.0;
+            for (int64_t p = 0; p < npieces[c]; ++p, ++pl)
+                for (int64_t i = 0; i < *pl; ++i, ++v) n2 += v[0] * v[0];
+            v = v0;
+            pl = pl0;
+            for (int64_t p = 0; p < npieces[c]; ++p, ++pl) {
+                double mx = 0.0;
+                for (int64_t i = 0; i < *pl; ++i) mx = fmax(mx, fabs(v[i]));
+                const int k = oracle_bfp_exp(mx, nb);
+                for (int64_t i = 0; i < *pl; ++i, ++v) {
+                    const double d = oracle_bfp_round(v[0], k, nb) - v[0];
+                    e2 += d * d;
+                }
+            }
+            if (!(e2 <= ldexp(n2, -2 * (rate + 1)))) ok = 0;
+        }
+        if (ok) return HMVM_FMT_B8 + f;
+    }
+    return rate <= 23 ? HMVM_FMT_F32 : HMVM_FMT_F64;
+}

@@ -16,7 +16,7 @@ def rel(a, b):
 def dequantized_arena(plan, hm):
     """FP64 arena holding exactly the values the plan stores (dequantised),
     at the generator's offsets, so the oracle can run on them."""
-    ar = [plan.arena_numpy(f) for f in range(4)]
+    ar = [plan.arena_numpy(f) for f in range(N.NFMT)]
     out = np.array(hm.arena, copy=True)
     for b, blk in enumerate(hm.blocks):
         m, n, k = int(blk["nrows"]), int(blk["ncols"]), int(blk["rank"])

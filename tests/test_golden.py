@@ -66,7 +66,7 @@ def test_layout_holds_integers_exactly(g, prec, force):
     of the stored factors (the plan applies it in FP64 to V^T x)."""
     p = Plan(g["blocks"], g["arena"], g["nrows"], g["ncols"], precision=prec, eps=1e-6, force_format=force,
              dense_prec="fp64" if prec == "fp64" else "fp32", device=-1)
-    ar = [p.arena_numpy(f) for f in range(4)]
+    ar = [p.arena_numpy(f) for f in range(N.NFMT)]
     for b in range(len(g["blocks"])):
         A, B = p.unpack_block(b, ar)
         blk = g["blocks"][b]

@@ -110,3 +110,15 @@ def test_batch_mixed_precision(cuda, members):
     b.check()
     _check(hms, nrhs, xs, ys, [1.0] * len(hms), 1e-6)
     b.close()
+
+
+def test_batch_block_floating_point(cuda, members):
+    hms, nrhs = members
+    plans = [Plan.from_hmatrix(hm, precision="bfp", eps=1e-6) for hm in hms]
+    b = Batch(plans, nrhs)
+    xs, ys = _vectors(hms, nrhs, cuda, 10)
+    print(f"[gpu] batch bfp n={[hm.n for hm in hms]} nrhs={nrhs}", flush=True)
+    b.apply(xs, ys)
+    b.check()
+    _check(hms, nrhs, xs, ys, [1.0] * len(hms), 1e-6)
+    b.close()

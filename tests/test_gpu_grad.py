@@ -68,7 +68,7 @@ def test_grad(cuda, case):
             ref = np.outer(r_h[ro:ro + m], x_h[co:co + nn])
             assert rel(gA, ref) <= 1e-12
         elif k:
-            U, V = plan.unpack_block(int(bi), [plan.arena_numpy(f) for f in range(4)])
+            U, V = plan.unpack_block(int(bi), [plan.arena_numpy(f) for f in range(N.NFMT)])
             t = V.T @ x_h[co:co + nn]
             s = U.T @ r_h[ro:ro + m]
             # t and s are accumulated in FP32 inside a panel for FP32 factors

@@ -47,7 +47,7 @@ def sphere():
 
 def test_fp64_layout_roundtrip(sphere):
     p = Plan.from_hmatrix(sphere, precision="fp64", device=-1)
-    ar = [p.arena_numpy(f) for f in range(4)]
+    ar = [p.arena_numpy(f) for f in range(N.NFMT)]
     for b, blk in enumerate(sphere.blocks):
         m, n, k = int(blk["nrows"]), int(blk["ncols"]), int(blk["rank"])
         A, B = p.unpack_block(b, ar)
@@ -67,8 +67,8 @@ def test_valr_chooser_and_quantisation(sphere, eps):
     value equals the oracle's rounding of the FP64 value (after the
     power-of-two scale for FP8), bit for bit."""
     p = Plan.from_hmatrix(sphere, precision="valr", eps=eps, device=-1)
-    ar = [p.arena_numpy(f) for f in range(4)]
-    counts = [0, 0, 0, 0]
+    ar = [p.arena_numpy(f) for f in range(N.NFMT)]
+    counts = [0] * N.NFMT
     rng = np.random.default_rng(3)
     for b, blk in enumerate(sphere.blocks):
         m, n, k = int(blk["nrows"]), int(blk["ncols"]), int(blk["rank"])
@@ -105,9 +105,75 @@ def test_valr_chooser_and_quantisation(sphere, eps):
     assert counts == p.info()["cols_by_format"]
 
 
+def _bfp_expected(ref, f):
+    if f >= N.FMT_B8:
+        return oracle.bfp_round_column(ref, N.FMT_BITS[f])
+    return ref.astype(np.float32).astype(np.float64) if f == N.FMT_F32 else ref
+
+
+@pytest.mark.parametrize("eps", [1e-4, 1e-6, 1e-8])
+def test_bfp_chooser_and_quantisation(sphere, eps):
+    """Block floating point plans: every column's width follows the oracle's
+    chooser (U and V of a low-rank column share one width, dense columns at
+    rate(eps)), and every stored value equals the oracle's rounding with one
+    step per 64-row panel column, bit for bit."""
+    p = Plan.from_hmatrix(sphere, precision="bfp", eps=eps, device=-1)
+    ar = [p.arena_numpy(f) for f in range(N.NFMT)]
+    counts = [0] * N.NFMT
+    rng = np.random.default_rng(5)
+    rate_eps = oracle.eps_to_rate(eps)
+    for b, blk in enumerate(sphere.blocks):
+        m, n, k = int(blk["nrows"]), int(blk["ncols"]), int(blk["rank"])
+        check = rng.random() < 0.1
+        if int(blk["kind"]) == N.BLOCK_DENSE:
+            if not check:
+                continue
+            D0 = sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T
+            D, _ = p.unpack_block(b, ar)
+            for j in range(n):
+                f = oracle.bfp_format([D0[:, j]], rate_eps)
+                assert np.array_equal(D[:, j], _bfp_expected(D0[:, j], f)), (b, j, f)
+            continue
+        U0 = sphere.arena[blk["off_a"]:blk["off_a"] + m * k].reshape(k, m).T
+        V0 = sphere.arena[blk["off_b"]:blk["off_b"] + n * k].reshape(k, n).T
+        S = sphere.arena[blk["off_s"]:blk["off_s"] + k]
+        fm = [oracle.bfp_format([U0[:, l], V0[:, l]], oracle.eps_to_rate(u))
+              for l, u in enumerate(oracle.valr_tolerances(S, eps))]
+        for f in fm:
+            counts[f] += 1
+        if not check:
+            continue
+        U, V = p.unpack_block(b, ar)
+        for l, f in enumerate(fm):
+            assert np.array_equal(U[:, l], _bfp_expected(U0[:, l], f)), (b, l, f)
+            assert np.array_equal(V[:, l], _bfp_expected(V0[:, l], f)), (b, l, f)
+    info = p.info()
+    assert counts == info["cols_by_format"]
+    # the block floating point widths carry the columns, and the plan is
+    # smaller than the mixed hardware-format plan at the same eps
+    assert sum(info["cols_by_format"][N.FMT_B8:]) > 0.8 * sum(counts)
+    q = Plan.from_hmatrix(sphere, precision="valr", eps=eps, device=-1)
+    assert info["factor_bytes"] < q.info()["factor_bytes"]
+    # bytes of each arena: 16 per row of 128 // bits values
+    for f in range(N.NFMT):
+        assert info["bytes_by_format"][f] == p.arena_elems(f) // (128 // N.FMT_BITS[f]) * 16
+
+
+def test_bfp_force_format_and_errors(sphere):
+    """force_format stores every low-rank column in one width; bad options
+    are rejected."""
+    for f in range(N.FMT_B8, N.NFMT):
+        p = Plan.from_hmatrix(sphere, precision="bfp", eps=1e-6, device=-1, force_format=("bfp8", "bfp10", "bfp12",
+                              "bfp16", "bfp21", "bfp25")[f - N.FMT_B8])
+        c = p.info()["cols_by_format"]
+        assert c[f] == sum(c) > 0
+    with pytest.raises(RuntimeError):
+        Plan.from_hmatrix(sphere, precision="bfp", eps=0.0, device=-1)
+
+
 def test_dense_and_datasize_valr(sphere):
     p = Plan.from_hmatrix(sphere, precision="valr", eps=1e-6, device=-1)
-    ar = [p.arena_numpy(f) for f in range(4)]
+    ar = [p.arena_numpy(f) for f in range(N.NFMT)]
     for b, blk in enumerate(sphere.blocks[:200]):
         if int(blk["kind"]) == N.BLOCK_DENSE:
             m, n = int(blk["nrows"]), int(blk["ncols"])
@@ -183,7 +249,7 @@ def test_dense_precision_follows_eps(sphere, eps, exact):
     """Dense leaves under the default rule: FP32 while ceil(-log2 eps) <= 23,
     FP64 below (FP32 rounding alone would exceed eps)."""
     p = Plan.from_hmatrix(sphere, precision="valr", eps=eps, device=-1)
-    ar = [p.arena_numpy(f) for f in range(4)]
+    ar = [p.arena_numpy(f) for f in range(N.NFMT)]
     assert (oracle.eps_to_rate(eps) > 23) == exact
     for b, blk in enumerate(sphere.blocks[:300]):
         if int(blk["kind"]) != N.BLOCK_DENSE:
@@ -194,7 +260,7 @@ def test_dense_precision_follows_eps(sphere, eps, exact):
         assert np.array_equal(D, ref if exact else ref.astype(np.float32).astype(np.float64))
     # an explicit request overrides the rule
     q = Plan.from_hmatrix(sphere, precision="valr", eps=eps, dense_prec="fp32", device=-1)
-    qa = [q.arena_numpy(f) for f in range(4)]
+    qa = [q.arena_numpy(f) for f in range(N.NFMT)]
     b = next(i for i, blk in enumerate(sphere.blocks) if int(blk["kind"]) == N.BLOCK_DENSE)
     D, _ = q.unpack_block(b, qa)
     blk = sphere.blocks[b]

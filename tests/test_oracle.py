@@ -211,3 +211,82 @@ def test_rounding_vs_torch():
         assert (math.isnan(d) and math.isnan(codes[c])) or d == codes[c]
     # saturation
     assert oracle.round_e4m3(463.9) == 448.0 and oracle.round_e4m3(1e6) == 448.0
+
+
+# ---------------------------------------------------------------------------
+# block floating point
+# ---------------------------------------------------------------------------
+
+BFP_BITS = (8, 10, 12, 16, 21, 25)
+
+
+@pytest.mark.parametrize("nb", BFP_BITS)
+def test_bfp_step_is_minimal(nb):
+    """The step 2^k is the smallest power of two with max|v| <= qmax 2^k:
+    brute force over k, on random magnitudes and on exact powers of two."""
+    rng = np.random.default_rng(nb)
+    qmax = 2.0 ** (nb - 1) - 1
+    mags = list(np.exp(rng.uniform(-60, 60, 200))) + [2.0 ** e for e in range(-40, 40, 7)] + [qmax, qmax * 2.0 ** -30]
+    for a in mags:
+        k = oracle.bfp_exp(a, nb)
+        assert a <= qmax * 2.0 ** k
+        assert a > qmax * 2.0 ** (k - 1)
+    assert oracle.bfp_exp(0.0, nb) == 0
+    # the step stays a normal FP32 number
+    assert oracle.bfp_exp(1e-300, nb) == -126
+
+
+@pytest.mark.parametrize("nb", BFP_BITS)
+def test_bfp_round_is_numpy_rint(nb):
+    """Rounding to the nearest multiple of the step, ties to even, is numpy's
+    rint (IEEE round-half-even) of v / 2^k, clamped to the NB-bit range."""
+    rng = np.random.default_rng(100 + nb)
+    qmax = 2.0 ** (nb - 1) - 1
+    for _ in range(300):
+        k = int(rng.integers(-40, 10))
+        v = rng.uniform(-qmax, qmax) * 2.0 ** k
+        assert oracle.bfp_round(v, k, nb) == np.rint(v / 2.0 ** k) * 2.0 ** k
+    # ties go to the even neighbour, beyond the range to +-qmax
+    assert oracle.bfp_round(2.5, 0, nb) == 2.0 and oracle.bfp_round(3.5, 0, nb) == 4.0
+    assert oracle.bfp_round(-2.5, 0, nb) == -2.0
+    assert oracle.bfp_round(10 * qmax, 0, nb) == qmax and oracle.bfp_round(-10 * qmax, 0, nb) == -qmax
+
+
+def _bfp_err(col, nb, piece=64):
+    return np.linalg.norm(oracle.bfp_round_column(col, nb, piece) - col)
+
+
+def test_bfp_format_is_the_narrowest_passing():
+    """Brute force: the chosen width is the first in 8..25 bits whose stored
+    columns (all of them) meet ||err|| <= 2^-(rate+1) ||col||, with one step per
+    64-row panel column; columns no width meets go to FP32 / FP64."""
+    rng = np.random.default_rng(7)
+    for trial in range(60):
+        rate = int(rng.integers(1, 30))
+        cols = [rng.standard_normal(int(rng.integers(1, 200))) * np.exp(rng.uniform(-5, 5, 1))
+                for _ in range(int(rng.integers(1, 3)))]
+        if trial % 3 == 0:   # a spread of magnitudes inside the column
+            cols[0] = cols[0] * np.exp(rng.uniform(-8, 0, len(cols[0])))
+        f = oracle.bfp_format(cols, rate)
+        ok = [all(_bfp_err(c, nb) <= 2.0 ** -(rate + 1) * np.linalg.norm(c) for c in cols) for nb in BFP_BITS]
+        if any(ok):
+            assert f == N.FMT_B8 + ok.index(True), (trial, rate, ok, f)
+        else:
+            assert f == (N.FMT_F32 if rate <= 23 else N.FMT_F64)
+
+
+def test_bfp_steps_are_per_panel_column():
+    """One step per panel column, not per column: a 64-row panel 10^6 times
+    smaller than its neighbour keeps its own relative accuracy, where a
+    single step for the whole column would flush it to zero."""
+    rng = np.random.default_rng(3)
+    col = np.concatenate([rng.standard_normal(64), 1e-6 * rng.standard_normal(64)])
+    got = oracle.bfp_round_column(col, 12)
+    assert np.linalg.norm(got[64:] - col[64:]) <= 2.0 ** -10 * np.linalg.norm(col[64:])
+    assert np.all(oracle.bfp_round_column(col, 12, piece=[128])[64:] == 0.0)
+    # exact small integers are stored exactly by the narrowest width
+    ints = np.arange(-50.0, 50.0)
+    assert oracle.bfp_format([ints], 60) == N.FMT_B8
+    assert np.array_equal(oracle.bfp_round_column(ints, 8), ints)
+    # an all-zero column needs nothing
+    assert oracle.bfp_format([np.zeros(70)], 40) == N.FMT_B8
