@@ -64,16 +64,29 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
 #pragma unroll
     for (int w = 0; w < W; ++w) res[lane * W + w] = acc[w];
     wave_sync();
+    // Column c of the pack is column c - W gofs[s] of segment s: its table
+    // index is vc0[s] - W gofs[s] + c, selected in registers from the pack's
+    // (uniform) segment bounds, so that all W table loads of a lane are
+    // issued together (no load waits on another).
     const int ncol = ngr * W;
-    for (int c = lane; c < ncol; c += 64) {
+    int32_t vd[W];
+    float vs[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int c = lane + 64 * k;
         const int gr = c / W;
-        int s = 0;
-        while (s + 1 < nseg && gr >= pk.gofs[s + 1]) ++s;
-        const int64_t vc = int64_t(pk.vc0[s]) + (c - pk.gofs[s] * W);
-        const int32_t d = P.vdest[vc];
-        double v = double(res[c]);
-        if (E::kScaled) v *= double(P.vscale[vc]);
-        if (d >= 0) Pb[d] = v;
+        int32_t vb = pk.vc0[0] - pk.gofs[0] * W;
+#pragma unroll
+        for (int t = 1; t < kPackSegs; ++t)
+            if (t < nseg && gr >= pk.gofs[t]) vb = pk.vc0[t] - pk.gofs[t] * W;
+        const bool on = c < ncol;
+        vd[k] = on ? P.vdest[int64_t(vb) + c] : -1;
+        vs[k] = (E::kScaled && on) ? P.vscale[int64_t(vb) + c] : 1.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int c = lane + 64 * k;
+        if (vd[k] >= 0) Pb[vd[k]] = double(res[c]) * double(vs[k]);
     }
 }
 
@@ -114,15 +127,12 @@ __device__ inline void chunk_stage(const DevPlan& P, const RChunk& C, double alp
 }
 
 template <typename T>
-__device__ inline double chunk_compute(const DevPlan& P, const RChunk& C, int m, const void* zbuf, int wave,
-                                       int lane) {
+__device__ inline double chunk_range(const DevPlan& P, const RChunk& C, int m, const void* zbuf, int gb, int ge,
+                                     int lane) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
     const A* zs = static_cast<const A*>(zbuf);
-    const int per = (C.ngroups + kWaves - 1) / kWaves;
-    const int gb = wave * per;
-    const int ge = min(gb + per, int(C.ngroups));
     A acc[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) acc[w] = A(0);
@@ -142,6 +152,15 @@ __device__ inline double chunk_compute(const DevPlan& P, const RChunk& C, int m,
 #pragma unroll
     for (int w = 0; w < W; ++w) r += double(acc[w]);
     return r;
+}
+
+// the wave's contiguous quarter of one chunk
+template <typename T>
+__device__ inline double chunk_compute(const DevPlan& P, const RChunk& C, int m, const void* zbuf, int wave,
+                                       int lane) {
+    const int per = (C.ngroups + kWaves - 1) / kWaves;
+    const int gb = wave * per;
+    return chunk_range<T>(P, C, m, zbuf, gb, min(gb + per, int(C.ngroups)), lane);
 }
 
 // kWhole: the panel's z values fit in dynamic LDS and are staged once;
