@@ -10,7 +10,7 @@ One step = one H-MVM y += 2 A x over the whole matrix.  The mixed-precision
 plan (per-column FP32/BF16/FP8-E4M3 chosen from singular values against eps,
 dense leaves FP32, FP32 accumulation inside panels) is the measured path;
 the plain FP64 plan is timed beside it, both with factors resident in HBM.
-A third plan stores every column in block floating point (values of 8 to 25
+A third plan stores every column in block floating point (values of 8, 16 or 25
 bits with a power-of-two step per panel column, width chosen per column so
 that the column's 2-norm error stays within the same VALR budget); its
 figures are reported beside the mixed-precision ones under "bfp".

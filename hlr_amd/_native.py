@@ -24,10 +24,10 @@ BLOCK_DENSE, BLOCK_LOWRANK, BLOCK_LOWRANK_SV = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_VALR, PREC_BFP = 0, 1, 2, 3
 F64, F32 = 0, 1
 FMT_F64, FMT_F32, FMT_BF16, FMT_FP8 = 0, 1, 2, 3
-FMT_B8, FMT_B10, FMT_B12, FMT_B16, FMT_B21, FMT_B25 = 4, 5, 6, 7, 8, 9
-NFMT = 10
+FMT_B8, FMT_B16, FMT_B25 = 4, 5, 6
+NFMT = 7
 # bits per stored value of each format; a 16-byte row holds 128 // bits values
-FMT_BITS = (64, 32, 16, 8, 8, 10, 12, 16, 21, 25)
+FMT_BITS = (64, 32, 16, 8, 8, 16, 25)
 
 KERNEL_LOG, KERNEL_LOG_GALERKIN, KERNEL_MATERN, KERNEL_INVR = 0, 1, 2, 3
 GEOM_LINE, GEOM_GRID1D, GEOM_CUBE, GEOM_SPHERE = 0, 1, 2, 3

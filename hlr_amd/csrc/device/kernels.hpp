@@ -133,7 +133,7 @@ template <> struct Elem<uint8_t> {    // FP8 E4M3 (OCP)
 // block floating point with NB-bit values (tag type; the arena holds packed rows)
 template <int NB> struct Bfp {};
 constexpr int bfp_format(int nb) {
-    return nb == 8 ? B8 : nb == 10 ? B10 : nb == 12 ? B12 : nb == 16 ? B16 : nb == 21 ? B21 : nb == 25 ? B25 : -1;
+    return nb == 8 ? B8 : nb == 16 ? B16 : nb == 25 ? B25 : -1;
 }
 
 template <int NB> struct Elem<Bfp<NB>> {
@@ -165,10 +165,7 @@ __device__ inline bytep elem_addr(const void* arena, int64_t off) {
     case BF16: { using T = uint16_t; __VA_ARGS__; } break;           \
     case FP8: { using T = uint8_t; __VA_ARGS__; } break;             \
     case B8: { using T = Bfp<8>; __VA_ARGS__; } break;               \
-    case B10: { using T = Bfp<10>; __VA_ARGS__; } break;             \
-    case B12: { using T = Bfp<12>; __VA_ARGS__; } break;             \
     case B16: { using T = Bfp<16>; __VA_ARGS__; } break;             \
-    case B21: { using T = Bfp<21>; __VA_ARGS__; } break;             \
     default: { using T = Bfp<25>; __VA_ARGS__; } break;              \
     }
 

@@ -31,13 +31,12 @@
 namespace hmvm {
 
 // Storage formats of factor columns.  F64, F32, BF16 and FP8 (E4M3) hold
-// W = 16 / sizeof(element) values per 16-byte row.  B8 .. B25 are block
+// W = 16 / sizeof(element) values per 16-byte row.  B8, B16, B25 are block
 // floating point: NB-bit two's-complement integers q, W = floor(128 / NB) of
 // them per 16-byte row (value j in bits [j NB, j NB + NB) of the row read as
 // a little-endian 128-bit integer), each standing for q * 2^k with one
 // power-of-two scale 2^k per panel column (as FP8's scale).
-enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, B8 = 4, B10 = 5, B12 = 6, B16 = 7, B21 = 8, B25 = 9,
-                     NFMT = 10 };
+enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, B8 = 4, B16 = 5, B25 = 6, NFMT = 7 };
 constexpr int kFirstBfp = B8;
 constexpr int kNumBfp = NFMT - kFirstBfp;
 
@@ -49,8 +48,7 @@ constexpr int32_t kPadCol = INT32_MIN;
 
 // bits per stored value, values per 16-byte row
 HMVM_HD constexpr int fmt_bits(int f) {
-    return f == F64 ? 64 : f == F32 ? 32 : f == BF16 ? 16 : f == FP8 ? 8 : f == B8 ? 8 : f == B10 ? 10
-         : f == B12 ? 12 : f == B16 ? 16 : f == B21 ? 21 : 25;
+    return f == F64 ? 64 : f == F32 ? 32 : f == BF16 ? 16 : f == FP8 ? 8 : f == B8 ? 8 : f == B16 ? 16 : 25;
 }
 HMVM_HD constexpr int fmt_width(int f) { return 128 / fmt_bits(f); }
 HMVM_HD constexpr bool fmt_is_bfp(int f) { return f >= kFirstBfp; }

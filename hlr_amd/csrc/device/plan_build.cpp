@@ -11,8 +11,8 @@
 //
 // HMVM_PREC_BFP stores columns as block floating point, as bfp::compress_lr
 // (compress/bfp.hh:765-860) does with one scale per column: here the scale
-// is a power of two per panel column and the value width NB comes from a
-// small set of packings (8, 10, 12, 16, 21, 25 bits).  The width of a
+// is a power of two per panel column and the value width NB is 8, 16 or 25
+// bits (16, 8 or 5 values per 16-byte row).  The width of a
 // column is the narrowest whose stored U and V columns are both within
 // 2^-(m_l+1) of the FP64 columns in the 2-norm -- the error a rounding
 // hardware format with m_l mantissa bits guarantees -- so the VALR error
@@ -102,9 +102,8 @@ int32_t bfp_quant(double v, int k, int nb) {
 namespace {
 
 // NB of each block floating point format, narrowest first
-constexpr int kBfpBits[kNumBfp] = {8, 10, 12, 16, 21, 25};
-static_assert(fmt_bits(B8) == 8 && fmt_bits(B10) == 10 && fmt_bits(B12) == 12 && fmt_bits(B16) == 16 &&
-              fmt_bits(B21) == 21 && fmt_bits(B25) == 25, "format table");
+constexpr int kBfpBits[kNumBfp] = {8, 16, 25};
+static_assert(fmt_bits(B8) == 8 && fmt_bits(B16) == 16 && fmt_bits(B25) == 25, "format table");
 
 // sum of squared errors of one panel column stored with NB-bit values
 double bfp_err2(const double* src, int64_t m, int nb) {
@@ -832,10 +831,7 @@ static double load_elem(int f, const void* const arenas[NFMT], int64_t idx, bool
     case BF16: return decode_bf16(static_cast<const uint16_t*>(arenas[BF16])[idx]);
     case FP8: return decode_e4m3(static_cast<const uint8_t*>(arenas[FP8])[idx]);
     case B8: return bfp_get<8>(w, idx);
-    case B10: return bfp_get<10>(w, idx);
-    case B12: return bfp_get<12>(w, idx);
     case B16: return bfp_get<16>(w, idx);
-    case B21: return bfp_get<21>(w, idx);
     default: return bfp_get<25>(w, idx);
     }
 }

@@ -14,7 +14,7 @@ import numpy as np
 from . import _native as N
 
 PRECISIONS = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "valr": N.PREC_VALR, "bfp": N.PREC_BFP}
-FORMAT_NAMES = ("fp64", "fp32", "bf16", "fp8_e4m3", "bfp8", "bfp10", "bfp12", "bfp16", "bfp21", "bfp25")
+FORMAT_NAMES = ("fp64", "fp32", "bf16", "fp8_e4m3", "bfp8", "bfp16", "bfp25")
 # bytes per stored value (16 bytes hold 128 // bits values)
 FORMAT_BYTES = tuple(16.0 / (128 // b) for b in N.FMT_BITS)
 

@@ -67,8 +67,8 @@ enum {
                              values against eps (compress/valr.hh:402-512,
                              compress/afl.hh:81-82,854-856); dense leaves FP32          */
     HMVM_PREC_BFP  = 3    /* per-column block floating point (compress/bfp.hh:765-860
-                             with VALR tolerances): the narrowest of the B8..B25
-                             formats whose stored column, U and V alike, is within
+                             with VALR tolerances): the narrowest of the B8, B16,
+                             B25 formats whose stored column, U and V alike, is within
                              2^-(rate+1) of the FP64 column in the 2-norm, rate from
                              the column's VALR tolerance; FP32 / FP64 when none is.
                              Dense leaves by the same rule at rate(eps) per column    */
@@ -109,13 +109,10 @@ enum {
        the row as a little-endian 128-bit integer), value q * 2^k with one
        power-of-two step 2^k per panel column                                 */
     HMVM_FMT_B8   = 4,    /* NB =  8, 16 values per row */
-    HMVM_FMT_B10  = 5,    /* NB = 10, 12 values per row */
-    HMVM_FMT_B12  = 6,    /* NB = 12, 10 values per row */
-    HMVM_FMT_B16  = 7,    /* NB = 16,  8 values per row */
-    HMVM_FMT_B21  = 8,    /* NB = 21,  6 values per row */
-    HMVM_FMT_B25  = 9     /* NB = 25,  5 values per row */
+    HMVM_FMT_B16  = 5,    /* NB = 16,  8 values per row */
+    HMVM_FMT_B25  = 6     /* NB = 25,  5 values per row */
 };
-#define HMVM_NFMT 10
+#define HMVM_NFMT 7
 
 typedef struct hmvm_plan_opts {
     int32_t precision;    /* HMVM_PREC_*                                          */

@@ -29,8 +29,8 @@
  *   oracle_round_*        round-to-nearest-even into BF16 / FP8-E4M3 (OCP)
  *   oracle_bfp_*          block floating point columns: bfp::compress_lr
  *                         (compress/bfp.hh:765-860) with a power-of-two step
- *                         per panel column and value widths of 8, 10, 12, 16,
- *                         21 or 25 bits; the width of a column is the
+ *                         per panel column and value widths of 8, 16 or 25
+ *                         bits; the width of a column is the
  *                         narrowest whose stored column is within
  *                         2^-(rate+1) of the column in the 2-norm
  *   oracle_dense_expand   sum of the leaves placed at their offsets
@@ -451,18 +451,18 @@ double oracle_bfp_round(double v, int k, int nb) {
     return ldexp(q, k);
 }
 
-/* value widths of the formats HMVM_FMT_B8 .. HMVM_FMT_B25 */
-static const int oracle_bfp_bits[6] = {8, 10, 12, 16, 21, 25};
+/* value widths of the formats HMVM_FMT_B8, HMVM_FMT_B16, HMVM_FMT_B25 */
+static const int oracle_bfp_bits[3] = {8, 16, 25};
 
 /*
  * Storage format of one column group: ncols columns, values concatenated in
  * `vals`; column c consists of npieces[c] panel columns whose lengths follow
- * one another in `plen`.  Returns the first of HMVM_FMT_B8 .. HMVM_FMT_B25
+ * one another in `plen`.  Returns the first of HMVM_FMT_B8, _B16, _B25
  * for which every column's stored pieces are within 2^-(rate+1) of the
  * column in the 2-norm, else HMVM_FMT_F32 for rate <= 23, else HMVM_FMT_F64.
  */
 int oracle_bfp_format(int ncols, const double* vals, const int64_t* npieces, const int64_t* plen, int rate) {
-    for (int f = 0; f < 6; ++f) {
+    for (int f = 0; f < 3; ++f) {
         const int nb = oracle_bfp_bits[f];
         int ok = 1;
         const double* v = vals;

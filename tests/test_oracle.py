@@ -217,7 +217,7 @@ def test_rounding_vs_torch():
 # block floating point
 # ---------------------------------------------------------------------------
 
-BFP_BITS = (8, 10, 12, 16, 21, 25)
+BFP_BITS = (8, 16, 25)
 
 
 @pytest.mark.parametrize("nb", BFP_BITS)
@@ -257,7 +257,7 @@ def _bfp_err(col, nb, piece=64):
 
 
 def test_bfp_format_is_the_narrowest_passing():
-    """Brute force: the chosen width is the first in 8..25 bits whose stored
+    """Brute force: the chosen width is the first of 8, 16, 25 bits whose stored
     columns (all of them) meet ||err|| <= 2^-(rate+1) ||col||, with one step per
     64-row panel column; columns no width meets go to FP32 / FP64."""
     rng = np.random.default_rng(7)
@@ -281,9 +281,9 @@ def test_bfp_steps_are_per_panel_column():
     single step for the whole column would flush it to zero."""
     rng = np.random.default_rng(3)
     col = np.concatenate([rng.standard_normal(64), 1e-6 * rng.standard_normal(64)])
-    got = oracle.bfp_round_column(col, 12)
+    got = oracle.bfp_round_column(col, 16)
     assert np.linalg.norm(got[64:] - col[64:]) <= 2.0 ** -10 * np.linalg.norm(col[64:])
-    assert np.all(oracle.bfp_round_column(col, 12, piece=[128])[64:] == 0.0)
+    assert np.all(oracle.bfp_round_column(col, 16, piece=[128])[64:] == 0.0)
     # exact small integers are stored exactly by the narrowest width
     ints = np.arange(-50.0, 50.0)
     assert oracle.bfp_format([ints], 60) == N.FMT_B8
