@@ -8,7 +8,7 @@
 // (programs/mvm/mvm.hh:166,178) unless --xrand is given.
 //
 // usage: mvm-hip [--app log|logkernel|materncov|invr] [-n N] [--ntile K]
-//                [--eta E] [--eps E] [--nbench B] [--prec fp64|fp32|valr]
+//                [--eta E] [--eps E] [--nbench B] [--prec fp64|fp32|valr|bfp]
 //                [--tlr] [--xrand] [--op normal|transposed]
 
 #include <algorithm>
@@ -68,9 +68,10 @@ int main(int argc, char** argv) {
         hlrgen_free(gm);
 
         hmvm_plan_opts opts{};
-        opts.precision = prec == "fp32" ? HMVM_PREC_FP32 : prec == "valr" ? HMVM_PREC_VALR : HMVM_PREC_FP64;
+        opts.precision = prec == "fp32" ? HMVM_PREC_FP32 : prec == "valr" ? HMVM_PREC_VALR
+                       : prec == "bfp" ? HMVM_PREC_BFP : HMVM_PREC_FP64;
         opts.eps = cfg.eps;
-        opts.dense_prec = HMVM_PREC_VALR;          // dense leaves FP32 unless eps needs FP64
+        opts.dense_prec = HMVM_PREC_VALR;          // dense leaves by eps (valr: FP32 unless eps needs FP64)
         opts.check_finite = 1;
         hmatrix A(blocks, arena, n, n, &opts, 0);
         std::printf("plan      : %s, %.3f GB of factors, %lld dense / %lld low-rank leaves\n", prec.c_str(),
