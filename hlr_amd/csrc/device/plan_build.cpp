@@ -12,11 +12,11 @@
 // HMVM_PREC_BFP stores columns as block floating point, as bfp::compress_lr
 // (compress/bfp.hh:765-860) does with one scale per column: here the scale
 // is a power of two per panel column and the value width NB is 8, 16 or 25
-// bits (16, 8 or 5 values per 16-byte row).  The width of a
-// column is the narrowest whose stored U and V columns are both within
-// 2^-(m_l+1) of the FP64 columns in the 2-norm -- the error a rounding
-// hardware format with m_l mantissa bits guarantees -- so the VALR error
-// budget holds as for the hardware formats.
+// bits (16, 8 or 5 values per 16-byte row).  The width of a low-rank column
+// is the narrowest whose stored U and V columns are both within the
+// column's VALR tolerance u_l of the FP64 columns in the 2-norm -- the
+// relative column error the VALR budget allows, which AFL's truncation to
+// m_l bits also stays within -- so the VALR error budget holds.
 
 #include "plan_host.hpp"
 
@@ -124,15 +124,16 @@ struct Piece {            // one panel column: m values from src
 };
 
 // the narrowest block floating point format whose stored pieces of every
-// column in `cols` are within 2^-(rate+1) of the column in the 2-norm; else
-// FP32 when rate <= 23 (its rounding meets the same bound), else FP64
-int bfp_choose(const std::vector<std::vector<Piece>>& cols, int rate) {
+// column in `cols` are within tol (relative, 2-norm) of the column; else
+// FP32 when rate <= 23 (its rounding error 2^-24 <= 2^-(rate+1) <= tol), else
+// FP64
+int bfp_choose(const std::vector<std::vector<Piece>>& cols, double tol, int rate) {
     std::vector<double> thr(cols.size());
     for (size_t c = 0; c < cols.size(); ++c) {
         double n2 = 0;
         for (const Piece& pc : cols[c])
             for (int64_t i = 0; i < pc.m; ++i) n2 += pc.src[i] * pc.src[i];
-        thr[c] = std::ldexp(n2, -2 * (rate + 1));
+        thr[c] = (tol * tol) * n2;
     }
     for (int f = 0; f < kNumBfp; ++f) {
         bool ok = true;
@@ -368,7 +369,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 for (int64_t q = bq0[i]; q < bq1[i]; ++q)
                     cols[1].push_back({arena + b.off_b + l * int64_t(b.ncols) + (cstart[q] - b.col_ofs),
                                        cpan[q].second});
-                L.tfmt[L.blk_t0[i] + l] = int8_t(bfp_choose(cols, eps_to_rate(tol[l])));
+                L.tfmt[L.blk_t0[i] + l] = int8_t(bfp_choose(cols, tol[l], eps_to_rate(tol[l])));
             }
         }
     }
@@ -381,13 +382,16 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             if (blocks[i].kind == HMVM_BLOCK_DENSE) { L.blk_d0[i] = nd; nd += blocks[i].ncols; }
         L.dfmt.assign(nd, int8_t(L.dense_fmt));
         if (dense_bfp) {
+            // a dense column has no VALR tolerance of its own: eps's rate plus
+            // one bit, the bound of a rounding format with rate(eps) bits
             const int rate = eps_to_rate(opts.eps);
+            const double dtol = std::ldexp(1.0, -(rate + 1));
 #pragma omp parallel for schedule(dynamic, 16)
             for (int64_t i = 0; i < nb; ++i) {
                 if (blocks[i].kind != HMVM_BLOCK_DENSE) continue;
                 for (int64_t j = 0; j < blocks[i].ncols; ++j) {
                     std::vector<std::vector<Piece>> cols(1, upieces(i, j));
-                    L.dfmt[L.blk_d0[i] + j] = int8_t(bfp_choose(cols, rate));
+                    L.dfmt[L.blk_d0[i] + j] = int8_t(bfp_choose(cols, dtol, rate));
                 }
             }
         }

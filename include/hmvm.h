@@ -69,9 +69,9 @@ enum {
     HMVM_PREC_BFP  = 3    /* per-column block floating point (compress/bfp.hh:765-860
                              with VALR tolerances): the narrowest of the B8, B16,
                              B25 formats whose stored column, U and V alike, is within
-                             2^-(rate+1) of the FP64 column in the 2-norm, rate from
-                             the column's VALR tolerance; FP32 / FP64 when none is.
-                             Dense leaves by the same rule at rate(eps) per column    */
+                             the column's VALR tolerance of the FP64 column in the
+                             2-norm; FP32 / FP64 when none is.  Dense leaves per
+                             column within 2^-(rate(eps)+1)                          */
 };
 
 /* vector element type at the boundary */

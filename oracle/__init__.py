@@ -67,7 +67,7 @@ def lib() -> C.CDLL:
         L.oracle_bfp_exp.restype = i32
         L.oracle_bfp_round.argtypes = [d, i32, i32]
         L.oracle_bfp_round.restype = d
-        L.oracle_bfp_format.argtypes = [i32, vp, vp, vp, i32]
+        L.oracle_bfp_format.argtypes = [i32, vp, vp, vp, d, i32]
         L.oracle_bfp_format.restype = i32
         _LIB = L
     return _LIB
@@ -183,10 +183,11 @@ def bfp_round_column(col, nb, piece=64):
     return out
 
 
-def bfp_format(cols, rate, piece=64):
+def bfp_format(cols, tol, rate, piece=64):
     """Format chosen for the columns `cols` (list of 1-D arrays) sharing one
-    width, each cut into panel columns of `piece` rows (or one list of piece
-    lengths per column)."""
+    width under relative 2-norm tolerance tol (fallback by rate), each cut
+    into panel columns of `piece` rows (or one list of piece lengths per
+    column)."""
     vals, npieces, plen = [], [], []
     for c, col in enumerate(cols):
         col = np.asarray(col, dtype=np.float64)
@@ -197,4 +198,4 @@ def bfp_format(cols, rate, piece=64):
     v = np.ascontiguousarray(np.concatenate(vals) if vals else np.zeros(0))
     npc = np.ascontiguousarray(npieces, dtype=np.int64)
     pl = np.ascontiguousarray(plen, dtype=np.int64)
-    return lib().oracle_bfp_format(len(cols), _p(v), _p(npc), _p(pl), int(rate))
+    return lib().oracle_bfp_format(len(cols), _p(v), _p(npc), _p(pl), float(tol), int(rate))

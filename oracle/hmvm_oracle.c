@@ -31,8 +31,9 @@
  *                         (compress/bfp.hh:765-860) with a power-of-two step
  *                         per panel column and value widths of 8, 16 or 25
  *                         bits; the width of a column is the
- *                         narrowest whose stored column is within
- *                         2^-(rate+1) of the column in the 2-norm
+ *                         narrowest whose stored column is within a
+ *                         given relative 2-norm tolerance (the VALR
+ *                         tolerance u_l of a low-rank column)
  *   oracle_dense_expand   sum of the leaves placed at their offsets
  *
  * Pins (tests/): dense expansion against direct kernel evaluation and
@@ -458,10 +459,11 @@ static const int oracle_bfp_bits[3] = {8, 16, 25};
  * Storage format of one column group: ncols columns, values concatenated in
  * `vals`; column c consists of npieces[c] panel columns whose lengths follow
  * one another in `plen`.  Returns the first of HMVM_FMT_B8, _B16, _B25
- * for which every column's stored pieces are within 2^-(rate+1) of the
+ * for which every column's stored pieces are within tol (relative) of the
  * column in the 2-norm, else HMVM_FMT_F32 for rate <= 23, else HMVM_FMT_F64.
  */
-int oracle_bfp_format(int ncols, const double* vals, const int64_t* npieces, const int64_t* plen, int rate) {
+int oracle_bfp_format(int ncols, const double* vals, const int64_t* npieces, const int64_t* plen, double tol,
+                      int rate) {
     for (int f = 0; f < 3; ++f) {
         const int nb = oracle_bfp_bits[f];
         int ok = 1;
@@ -484,7 +486,7 @@ int oracle_bfp_format(int ncols, const double* vals, const int64_t* npieces, con
                     e2 += d * d;
                 }
             }
-            if (!(e2 <= ldexp(n2, -2 * (rate + 1)))) ok = 0;
+            if (!(e2 <= (tol * tol) * n2)) ok = 0;
         }
         if (ok) return HMVM_FMT_B8 + f;
     }

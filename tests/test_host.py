@@ -114,9 +114,10 @@ def _bfp_expected(ref, f):
 @pytest.mark.parametrize("eps", [1e-4, 1e-6, 1e-8])
 def test_bfp_chooser_and_quantisation(sphere, eps):
     """Block floating point plans: every column's width follows the oracle's
-    chooser (U and V of a low-rank column share one width, dense columns at
-    rate(eps)), and every stored value equals the oracle's rounding with one
-    step per 64-row panel column, bit for bit."""
+    chooser (U and V of a low-rank column share one width within the column's
+    VALR tolerance, dense columns within 2^-(rate(eps)+1)), and every stored
+    value equals the oracle's rounding with one step per 64-row panel column,
+    bit for bit."""
     p = Plan.from_hmatrix(sphere, precision="bfp", eps=eps, device=-1)
     ar = [p.arena_numpy(f) for f in range(N.NFMT)]
     counts = [0] * N.NFMT
@@ -131,13 +132,13 @@ def test_bfp_chooser_and_quantisation(sphere, eps):
             D0 = sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T
             D, _ = p.unpack_block(b, ar)
             for j in range(n):
-                f = oracle.bfp_format([D0[:, j]], rate_eps)
+                f = oracle.bfp_format([D0[:, j]], 2.0 ** -(rate_eps + 1), rate_eps)
                 assert np.array_equal(D[:, j], _bfp_expected(D0[:, j], f)), (b, j, f)
             continue
         U0 = sphere.arena[blk["off_a"]:blk["off_a"] + m * k].reshape(k, m).T
         V0 = sphere.arena[blk["off_b"]:blk["off_b"] + n * k].reshape(k, n).T
         S = sphere.arena[blk["off_s"]:blk["off_s"] + k]
-        fm = [oracle.bfp_format([U0[:, l], V0[:, l]], oracle.eps_to_rate(u))
+        fm = [oracle.bfp_format([U0[:, l], V0[:, l]], u, oracle.eps_to_rate(u))
               for l, u in enumerate(oracle.valr_tolerances(S, eps))]
         for f in fm:
             counts[f] += 1

@@ -258,19 +258,20 @@ def _bfp_err(col, nb, piece=64):
 
 def test_bfp_format_is_the_narrowest_passing():
     """Brute force: the chosen width is the first of 8, 16, 25 bits whose stored
-    columns (all of them) meet ||err|| <= 2^-(rate+1) ||col||, with one step per
-    64-row panel column; columns no width meets go to FP32 / FP64."""
+    columns (all of them) meet ||err|| <= tol ||col||, with one step per 64-row
+    panel column; columns no width meets go to FP32 / FP64 by rate."""
     rng = np.random.default_rng(7)
     for trial in range(60):
-        rate = int(rng.integers(1, 30))
+        tol = float(np.exp2(-rng.uniform(1, 30)))
+        rate = max(1, int(np.ceil(-np.log2(tol))))
         cols = [rng.standard_normal(int(rng.integers(1, 200))) * np.exp(rng.uniform(-5, 5, 1))
                 for _ in range(int(rng.integers(1, 3)))]
         if trial % 3 == 0:   # a spread of magnitudes inside the column
             cols[0] = cols[0] * np.exp(rng.uniform(-8, 0, len(cols[0])))
-        f = oracle.bfp_format(cols, rate)
-        ok = [all(_bfp_err(c, nb) <= 2.0 ** -(rate + 1) * np.linalg.norm(c) for c in cols) for nb in BFP_BITS]
+        f = oracle.bfp_format(cols, tol, rate)
+        ok = [all(_bfp_err(c, nb) <= tol * np.linalg.norm(c) for c in cols) for nb in BFP_BITS]
         if any(ok):
-            assert f == N.FMT_B8 + ok.index(True), (trial, rate, ok, f)
+            assert f == N.FMT_B8 + ok.index(True), (trial, tol, ok, f)
         else:
             assert f == (N.FMT_F32 if rate <= 23 else N.FMT_F64)
 
@@ -286,7 +287,7 @@ def test_bfp_steps_are_per_panel_column():
     assert np.all(oracle.bfp_round_column(col, 16, piece=[128])[64:] == 0.0)
     # exact small integers are stored exactly by the narrowest width
     ints = np.arange(-50.0, 50.0)
-    assert oracle.bfp_format([ints], 60) == N.FMT_B8
+    assert oracle.bfp_format([ints], 2.0 ** -60, 60) == N.FMT_B8
     assert np.array_equal(oracle.bfp_round_column(ints, 8), ints)
     # an all-zero column needs nothing
-    assert oracle.bfp_format([np.zeros(70)], 40) == N.FMT_B8
+    assert oracle.bfp_format([np.zeros(70)], 2.0 ** -40, 40) == N.FMT_B8
