@@ -2,8 +2,9 @@
   - Matern nu = 1.5 on n = 65536 points in the unit cube, FP64 plan, normal
     and transposed products: relative error <= 1e-13
   - 1/r on n = 262144 points on the unit sphere, eps in {1e-4, 1e-6, 1e-8}:
-    FP64 plan <= 1e-13, mixed-precision plan <= eps (normal and
-    transposed), and 32 right-hand sides at eps = 1e-6 on the matrix cores
+    FP64 plan <= 1e-13, mixed-precision and block floating point plans
+    <= eps (normal and transposed), and 32 right-hand sides at eps = 1e-6
+    on the matrix cores
   - tile-low-rank Matern on n = 131072 (tile 512, FP32 factors): gradient
     of L = 1/2 ||A x - b||^2 with respect to x against the oracle's
     transposed product on the stored factors, and the gradients with
@@ -59,18 +60,22 @@ def test_sphere_eps_sweep(cuda, eps):
     print(f"[gpu] fullsize sphere n={hm.n} eps={eps:g} leaves={len(hm.blocks)}", flush=True)
     p64 = Plan.from_hmatrix(hm, precision="fp64")
     pmp = Plan.from_hmatrix(hm, precision="valr", eps=eps)
-    info = pmp.info()
+    pbf = Plan.from_hmatrix(hm, precision="bfp", eps=eps)
+    info, ibf = pmp.info(), pbf.info()
     for op in (N.OP_NORMAL, N.OP_TRANSPOSED):
         x = random_vector(hm.n, 2 + op)
         y0 = np.zeros(hm.n)
         yr = _ref(hm, 1.0, op, x, y0)
         e64 = rel(_gpu(p64, 1.0, op, x, y0, cuda), yr)
         emp = rel(_gpu(pmp, 1.0, op, x, y0, cuda), yr)
+        ebf = rel(_gpu(pbf, 1.0, op, x, y0, cuda), yr)
         print(f"[gpu] sphere n={hm.n} eps={eps:g} op={op} fp64 rel.err={e64:.3e} mixed rel.err={emp:.3e} "
-              f"cols/format={info['cols_by_format']} bytes fp64={p64.info()['datasize']:.4g} "
-              f"mixed={info['datasize']:.4g}", flush=True)
+              f"bfp rel.err={ebf:.3e} cols/format mixed={info['cols_by_format']} bfp={ibf['cols_by_format']} "
+              f"bytes fp64={p64.info()['datasize']:.4g} mixed={info['datasize']:.4g} bfp={ibf['datasize']:.4g}",
+              flush=True)
         assert e64 <= 1e-13
         assert emp <= eps
+        assert ebf <= eps
 
 
 def test_sphere_32_rhs(cuda):
