@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -64,7 +66,7 @@ struct BatchUnit {
     int32_t mat, first, count, pad_;
 };
 
-template <typename XT>
+template <typename XT, int FAM>
 __global__ __launch_bounds__(kThreads) void k_bv1(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     extern __shared__ double lds[];
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(kThreads) void k_bv1(const BatchMat* __restrict__ m
     const VPack* pk = M.D.vpacks + w.first + wave;
     const XT* x = static_cast<const XT*>(M.x);
     double* xs = lds + wave * kPackSegs * kXsStride;
-    HMVM_FMT_SWITCH(pk->fmt, vpack_fwd<T>(M.D, pk, x, M.part, xs, lane))
+    HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(M.D, pk, x, M.part, xs, lane))
 }
 
 template <typename XT>
@@ -87,14 +89,14 @@ __global__ __launch_bounds__(kThreads) void k_bxt(const BatchMat* __restrict__ m
     xtrans_tile(static_cast<const XT*>(M.x), M.ldx, M.D.ncols, M.R, M.xt, int64_t(w.first) * 64, lds, threadIdx.x);
 }
 
-template <int NTM>
+template <int NTM, int FAM>
 __global__ __launch_bounds__(kThreads) void k_bvm(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
-    vseg_mm_dispatch<NTM>(M.D, M.D.vitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
+    vseg_mm_dispatch<NTM, FAM>(M.D, M.D.vitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
 }
 
 __global__ __launch_bounds__(kThreads) void k_bp1(const BatchMat* __restrict__ mats,
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void k_bp2(const BatchMat* __restrict__ m
     for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads) psum2_item(M.D, M.tmp, M.t, M.R, M.alpha, g);
 }
 
-template <typename XT, typename YT>
+template <typename XT, typename YT, int FAM>
 __global__ __launch_bounds__(kThreads) void k_br1(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     extern __shared__ double lds[];
@@ -122,7 +124,8 @@ __global__ __launch_bounds__(kThreads) void k_br1(const BatchMat* __restrict__ m
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
     const RowPanel Rw = M.D.rp[w.first];
-    const double acc = rpanel_fwd_chunked(M.D, Rw, M.alpha, static_cast<const XT*>(M.x), M.t, lds, tid, wave, lane);
+    const double acc = rpanel_fwd_chunked<XT, FAM>(M.D, Rw, M.alpha, static_cast<const XT*>(M.x), M.t, lds, tid, wave,
+                                                   lane);
     red[wave][lane] = acc;
     __syncthreads();
     if (wave == 0 && lane < Rw.m) {
@@ -132,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void k_br1(const BatchMat* __restrict__ m
     }
 }
 
-template <int NTM, typename YT>
+template <int NTM, typename YT, int FAM>
 __global__ __launch_bounds__(kThreads) void k_brm(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     extern __shared__ double lds[];
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kThreads) void k_brm(const BatchMat* __restrict__ m
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
     const RowPanel Rw = M.D.rp[w.first];
-    rpanel_mm<NTM>(M.D, Rw, M.alpha, M.xt, M.t, static_cast<YT*>(M.y), M.ldy, M.R, lds, tid, wave, lane);
+    rpanel_mm<NTM, YT, FAM>(M.D, Rw, M.alpha, M.xt, M.t, static_cast<YT*>(M.y), M.ldy, M.R, lds, tid, wave, lane);
 }
 
 inline int64_t pad32(int64_t v) { return ((v + 31) / 32) * 32; }
@@ -163,6 +166,7 @@ struct hmvm_batch {
     BatchUnit* d_units[kConfigs] = {};
     size_t lds[kKinds] = {};
     int config = 1;
+    int family = kFamAll;                   // the members' common format family, else all formats
     double* d_ws = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -200,35 +204,44 @@ int fill_descriptors(hmvm_batch* b, const double* alphas, const void* const* xs,
     return HMVM_OK;
 }
 
-template <typename XT, typename YT>
+template <typename XT, typename YT, int FAM>
 void launch_kinds(hmvm_batch* b, hipStream_t st) {
     const int c = b->config;
     const BatchUnit* u = b->d_units[c];
     const int64_t* o = b->kind_ofs[c];
     auto grid = [&](int k) { return unsigned(o[k + 1] - o[k]); };
     if (grid(K_XT)) k_bxt<XT><<<grid(K_XT), kThreads, b->lds[K_XT], st>>>(b->d_mats, u + o[K_XT]);
-    if (grid(K_V1)) k_bv1<XT><<<grid(K_V1), kThreads, b->lds[K_V1], st>>>(b->d_mats, u + o[K_V1]);
-    if (grid(K_VM1)) k_bvm<1><<<grid(K_VM1), kThreads, b->lds[K_VM1], st>>>(b->d_mats, u + o[K_VM1]);
-    if (grid(K_VM2)) k_bvm<2><<<grid(K_VM2), kThreads, b->lds[K_VM2], st>>>(b->d_mats, u + o[K_VM2]);
-    if (grid(K_VM3)) k_bvm<3><<<grid(K_VM3), kThreads, b->lds[K_VM3], st>>>(b->d_mats, u + o[K_VM3]);
-    if (grid(K_VM4)) k_bvm<4><<<grid(K_VM4), kThreads, b->lds[K_VM4], st>>>(b->d_mats, u + o[K_VM4]);
+    if (grid(K_V1)) k_bv1<XT, FAM><<<grid(K_V1), kThreads, b->lds[K_V1], st>>>(b->d_mats, u + o[K_V1]);
+    if (grid(K_VM1)) k_bvm<1, FAM><<<grid(K_VM1), kThreads, b->lds[K_VM1], st>>>(b->d_mats, u + o[K_VM1]);
+    if (grid(K_VM2)) k_bvm<2, FAM><<<grid(K_VM2), kThreads, b->lds[K_VM2], st>>>(b->d_mats, u + o[K_VM2]);
+    if (grid(K_VM3)) k_bvm<3, FAM><<<grid(K_VM3), kThreads, b->lds[K_VM3], st>>>(b->d_mats, u + o[K_VM3]);
+    if (grid(K_VM4)) k_bvm<4, FAM><<<grid(K_VM4), kThreads, b->lds[K_VM4], st>>>(b->d_mats, u + o[K_VM4]);
     if (grid(K_P1)) k_bp1<<<grid(K_P1), kThreads, 0, st>>>(b->d_mats, u + o[K_P1]);
     if (grid(K_P2)) k_bp2<<<grid(K_P2), kThreads, 0, st>>>(b->d_mats, u + o[K_P2]);
-    if (grid(K_R1)) k_br1<XT, YT><<<grid(K_R1), kThreads, b->lds[K_R1], st>>>(b->d_mats, u + o[K_R1]);
-    if (grid(K_RM1)) k_brm<1, YT><<<grid(K_RM1), kThreads, b->lds[K_RM1], st>>>(b->d_mats, u + o[K_RM1]);
-    if (grid(K_RM2)) k_brm<2, YT><<<grid(K_RM2), kThreads, b->lds[K_RM2], st>>>(b->d_mats, u + o[K_RM2]);
-    if (grid(K_RM3)) k_brm<3, YT><<<grid(K_RM3), kThreads, b->lds[K_RM3], st>>>(b->d_mats, u + o[K_RM3]);
-    if (grid(K_RM4)) k_brm<4, YT><<<grid(K_RM4), kThreads, b->lds[K_RM4], st>>>(b->d_mats, u + o[K_RM4]);
+    if (grid(K_R1)) k_br1<XT, YT, FAM><<<grid(K_R1), kThreads, b->lds[K_R1], st>>>(b->d_mats, u + o[K_R1]);
+    if (grid(K_RM1)) k_brm<1, YT, FAM><<<grid(K_RM1), kThreads, b->lds[K_RM1], st>>>(b->d_mats, u + o[K_RM1]);
+    if (grid(K_RM2)) k_brm<2, YT, FAM><<<grid(K_RM2), kThreads, b->lds[K_RM2], st>>>(b->d_mats, u + o[K_RM2]);
+    if (grid(K_RM3)) k_brm<3, YT, FAM><<<grid(K_RM3), kThreads, b->lds[K_RM3], st>>>(b->d_mats, u + o[K_RM3]);
+    if (grid(K_RM4)) k_brm<4, YT, FAM><<<grid(K_RM4), kThreads, b->lds[K_RM4], st>>>(b->d_mats, u + o[K_RM4]);
 }
 
 int launch(hmvm_batch* b, hipStream_t st) {
     if (hipMemcpyAsync(b->d_mats, b->h_mats, sizeof(BatchMat) * size_t(b->nmat), hipMemcpyHostToDevice, st) !=
         hipSuccess)
         return HMVM_ERR_DEVICE;
-    if (b->vdt == HMVM_F64)
-        launch_kinds<double, double>(b, st);
-    else
-        launch_kinds<float, float>(b, st);
+    // the matrix-core kernels come in one instantiation per format family
+    auto go = [&](auto fam) {
+        constexpr int FAM = decltype(fam)::value;
+        if (b->vdt == HMVM_F64)
+            launch_kinds<double, double, FAM>(b, st);
+        else
+            launch_kinds<float, float, FAM>(b, st);
+    };
+    switch (b->family) {
+    case kFamHw: go(std::integral_constant<int, kFamHw>()); break;
+    case kFamBfp: go(std::integral_constant<int, kFamBfp>()); break;
+    default: go(std::integral_constant<int, kFamAll>()); break;
+    }
     return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
 }
 
@@ -276,6 +289,9 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
     b->nmat = nplans;
     b->plans.assign(plans, plans + nplans);
     for (int i = 0; i < nplans; ++i) b->nrhs.push_back(nrhs ? nrhs[i] : 1);
+    b->family = nplans > 0 ? plan_layout(plans[0]).family : kFamHw;
+    for (int i = 1; i < nplans; ++i)
+        if (plan_layout(plans[i]).family != b->family) b->family = kFamAll;
     Guard g(dev);
     if (!g.ok) return HMVM_ERR_DEVICE;
 

@@ -169,7 +169,7 @@ __device__ inline double chunk_compute(const DevPlan& P, const RChunk& C, int m,
 // Whole row panel with chunk-by-chunk staging into a buffer of
 // kChunkGroups * 16 floats; all kThreads threads call it; returns the
 // lane's partial row sum (rows = lanes, waves split each chunk).
-template <typename XT>
+template <typename XT, int FAM = kFamAll>
 __device__ inline double rpanel_fwd_chunked(const DevPlan& P, const RowPanel& R, double alpha,
                                             const XT* __restrict__ x, const double* __restrict__ t, void* zbuf,
                                             int tid, int wave, int lane) {
@@ -177,9 +177,9 @@ __device__ inline double rpanel_fwd_chunked(const DevPlan& P, const RowPanel& R,
     for (int ci = 0; ci < R.nchunks; ++ci) {
         const RChunk C = P.rchunks[R.chunk0 + ci];
         __syncthreads();
-        HMVM_FMT_SWITCH(C.fmt, chunk_stage<T>(P, C, alpha, x, t, zbuf, tid))
+        HMVM_FMT_SWITCH_FAM(FAM, C.fmt, chunk_stage<T>(P, C, alpha, x, t, zbuf, tid))
         __syncthreads();
-        HMVM_FMT_SWITCH(C.fmt, acc += chunk_compute<T>(P, C, R.m, zbuf, wave, lane))
+        HMVM_FMT_SWITCH_FAM(FAM, C.fmt, acc += chunk_compute<T>(P, C, R.m, zbuf, wave, lane))
     }
     return acc;
 }

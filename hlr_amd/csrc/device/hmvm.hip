@@ -22,6 +22,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "hmvm.h"
@@ -40,14 +41,16 @@ namespace {
 // forward: V pass, one wave per pack of consecutive segments (lane = group)
 // ---------------------------------------------------------------------------
 
-template <typename XT>
-__global__ __launch_bounds__(kThreads) void k_vfwd(DevPlan P, const XT* __restrict__ x, double* __restrict__ Pb) {
+// at least 4 waves per SIMD (at most 128 VGPRs), the occupancy the pack
+// stream was measured at
+template <typename XT, int FAM>
+__global__ __launch_bounds__(kThreads, 4) void k_vfwd(DevPlan P, const XT* __restrict__ x, double* __restrict__ Pb) {
     __shared__ double xs[kWaves][kPackSegs * kXsStride];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t pi = int64_t(blockIdx.x) * kWaves + wave;
     if (pi >= P.nvpacks) return;
     const VPack* pk = P.vpacks + pi;
-    HMVM_FMT_SWITCH(pk->fmt, vpack_fwd<T>(P, pk, x, Pb, xs[wave], lane))
+    HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(P, pk, x, Pb, xs[wave], lane))
 }
 
 // stage 1: tmp[s] = sum of kSegLen consecutive partials of entry se[s]
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(256) void k_psum2(const int32_t* __restrict__ pos0,
 // each wave streams a contiguous quarter of the chunk's groups
 // ---------------------------------------------------------------------------
 
-template <typename XT, typename YT, bool kWhole>
+template <typename XT, typename YT, bool kWhole, int FAM>
 __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, const XT* __restrict__ x,
                                                    const double* __restrict__ t, YT* __restrict__ y) {
     extern __shared__ __align__(16) unsigned char zdyn[];
@@ -100,16 +103,16 @@ __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, cons
         for (int ci = 0; ci < R.nchunks; ++ci) {
             const RChunk C = P.rchunks[R.chunk0 + ci];
             void* zb = zdyn + C.zoff;
-            HMVM_FMT_SWITCH(C.fmt, chunk_stage<T>(P, C, alpha, x, t, zb, tid))
+            HMVM_FMT_SWITCH_FAM(FAM, C.fmt, chunk_stage<T>(P, C, alpha, x, t, zb, tid))
         }
         __syncthreads();
         for (int ci = 0; ci < R.nchunks; ++ci) {
             const RChunk C = P.rchunks[R.chunk0 + ci];
             const void* zb = zdyn + C.zoff;
-            HMVM_FMT_SWITCH(C.fmt, acc += chunk_compute<T>(P, C, R.m, zb, wave, lane))
+            HMVM_FMT_SWITCH_FAM(FAM, C.fmt, acc += chunk_compute<T>(P, C, R.m, zb, wave, lane))
         }
     } else {
-        acc = rpanel_fwd_chunked(P, R, alpha, x, t, zdyn, tid, wave, lane);
+        acc = rpanel_fwd_chunked<XT, FAM>(P, R, alpha, x, t, zdyn, tid, wave, lane);
     }
     red[wave][lane] = acc;
     __syncthreads();
@@ -249,7 +252,7 @@ __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VIte
     }
 }
 
-template <typename YT>
+template <typename YT, int FAM>
 __global__ __launch_bounds__(kThreads) void k_vtrn(DevPlan P, double alpha, const double* __restrict__ s,
                                                    const double* __restrict__ Q, YT* __restrict__ y) {
     __shared__ double red[kWaves][64];
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void k_vtrn(DevPlan P, double alpha, cons
     red[wave][lane] = 0.0;          // the wave's row sums; only this wave touches its row
     for (int j = wave; j < C.nitems; j += kWaves) {
         const VItem it = P.vitems[C.item0 + j];
-        HMVM_FMT_SWITCH(it.fmt, item_vtrn<T>(P, C, it, s, red[wave], lane))
+        HMVM_FMT_SWITCH_FAM(FAM, it.fmt, item_vtrn<T>(P, C, it, s, red[wave], lane))
     }
     __syncthreads();
     if (wave == 0 && lane < C.m) {
@@ -443,6 +446,13 @@ int upload_plan(hmvm_plan* p) {
     return HMVM_OK;
 }
 
+// calls f(std::integral_constant<int, FAM>) for the plan's format family
+template <typename F>
+void by_family(int fam, F&& f) {
+    if (fam == kFamBfp) f(std::integral_constant<int, kFamBfp>());
+    else f(std::integral_constant<int, kFamHw>());
+}
+
 inline unsigned blocks_for(int64_t n, int per) { return unsigned((n + per - 1) / per); }
 
 inline int64_t pad32(int64_t v) { return ((v + 31) / 32) * 32; }
@@ -482,17 +492,24 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
     double* t = tmp + pad32(nseg);
     if (ev) (void)hipEventRecord(ev[0], st);
     if (!trans) {
-        if (D.nvpacks > 0) k_vfwd<XT><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, x, part);
+        if (D.nvpacks > 0)
+            by_family(p->L.family, [&](auto fam) {
+                k_vfwd<XT, decltype(fam)::value><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, x, part);
+            });
         if (ev) (void)hipEventRecord(ev[1], st);
         launch_psum(D, false, part, tmp, t, alpha, st);
         if (ev) (void)hipEventRecord(ev[2], st);
         if (D.nrp > 0) {
             // chunk staging needs kChunkGroups * 16 floats (FP8) or * 2 doubles (FP64): 16 KB
-            if (p->L.max_zbytes <= kWholeZBytes)
-                k_ufwd<XT, YT, true><<<unsigned(D.nrp), kThreads, size_t(p->L.max_zbytes), st>>>(D, alpha, x, t, y);
-            else
-                k_ufwd<XT, YT, false><<<unsigned(D.nrp), kThreads, size_t(kChunkGroups * 16 * 4), st>>>(D, alpha, x, t,
-                                                                                                          y);
+            by_family(p->L.family, [&](auto fam) {
+                constexpr int FAM = decltype(fam)::value;
+                if (p->L.max_zbytes <= kWholeZBytes)
+                    k_ufwd<XT, YT, true, FAM><<<unsigned(D.nrp), kThreads, size_t(p->L.max_zbytes), st>>>(D, alpha, x,
+                                                                                                          t, y);
+                else
+                    k_ufwd<XT, YT, false, FAM><<<unsigned(D.nrp), kThreads, size_t(kChunkGroups * 16 * 4), st>>>(
+                        D, alpha, x, t, y);
+            });
         }
         if (ev) (void)hipEventRecord(ev[3], st);
     } else {
@@ -506,7 +523,10 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
         if (ev) (void)hipEventRecord(ev[1], st);
         launch_psum(D, true, part, tmp, t, alpha, st);
         if (ev) (void)hipEventRecord(ev[2], st);
-        if (D.ncp > 0) k_vtrn<YT><<<unsigned(D.ncp), kThreads, 0, st>>>(D, alpha, t, part, y);
+        if (D.ncp > 0)
+            by_family(p->L.family, [&](auto fam) {
+                k_vtrn<YT, decltype(fam)::value><<<unsigned(D.ncp), kThreads, 0, st>>>(D, alpha, t, part, y);
+            });
         if (ev) (void)hipEventRecord(ev[3], st);
     }
     return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
@@ -543,7 +563,7 @@ int apply_impl(const hmvm_plan* p, double alpha, int op, const void* x, int64_t 
                          : launch_apply(p, alpha, op, static_cast<const float*>(xr), static_cast<float*>(yr),
                                         static_cast<double*>(ws), st);
             } else {
-                rc = launch_apply_mrhs(p->dp, 0, alpha, xr, ldx, yr, ldy, R, vdt, static_cast<double*>(ws), st) == 0
+                rc = launch_apply_mrhs(p->dp, p->L.family, alpha, xr, ldx, yr, ldy, R, vdt, static_cast<double*>(ws), st) == 0
                          ? HMVM_OK : HMVM_ERR_DEVICE;
             }
             if (rc != HMVM_OK) return rc;
@@ -767,7 +787,10 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
         const XT* xx = static_cast<const XT*>(x);
         const XT* yb = static_cast<const XT*>(ybar);
         // t = S V^T x, s = S U^T ybar (alpha = 1)
-        if (D.nvpacks > 0) k_vfwd<XT><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, xx, Pb);
+        if (D.nvpacks > 0)
+            by_family(p->L.family, [&](auto fam) {
+                k_vfwd<XT, decltype(fam)::value><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, xx, Pb);
+            });
         for (int f = 0; f < NFMT; ++f) {
             const int64_t n = p->rtitem_start[f + 1] - p->rtitem_start[f];
             if (n == 0) continue;
@@ -777,7 +800,11 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
         }
         launch_psum(D, false, Pb, tmp, t, 1.0, st);
         launch_psum(D, true, Qb, tmp, s, 1.0, st);
-        if (gx && D.ncp > 0) k_vtrn<XT><<<unsigned(D.ncp), kThreads, 0, st>>>(D, 1.0, s, Qb, static_cast<XT*>(gx));
+        if (gx && D.ncp > 0)
+            by_family(p->L.family, [&](auto fam) {
+                k_vtrn<XT, decltype(fam)::value><<<unsigned(D.ncp), kThreads, 0, st>>>(D, 1.0, s, Qb,
+                                                                                       static_cast<XT*>(gx));
+            });
         if (grad_arena) {
             for (int f = 0; f < NFMT; ++f) {
                 if (!grad_arena[f]) continue;

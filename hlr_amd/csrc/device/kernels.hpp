@@ -157,6 +157,19 @@ __device__ inline bytep elem_addr(const void* arena, int64_t off) {
     return static_cast<bytep>(arena) + (off / Elem<T>::W) * 16;
 }
 
+// Like HMVM_FMT_SWITCH, restricted to family FAM (a compile-time constant);
+// formats outside it do nothing.
+#define HMVM_FMT_SWITCH_FAM(FAM, f, ...)                                                  \
+    switch (f) {                                                                          \
+    case F64: { using T = double; __VA_ARGS__; } break;                                   \
+    case F32: { using T = float; __VA_ARGS__; } break;                                    \
+    case BF16: if constexpr (FAM != kFamBfp) { using T = uint16_t; __VA_ARGS__; } break;  \
+    case FP8: if constexpr (FAM != kFamBfp) { using T = uint8_t; __VA_ARGS__; } break;    \
+    case B8: if constexpr (FAM != kFamHw) { using T = Bfp<8>; __VA_ARGS__; } break;       \
+    case B16: if constexpr (FAM != kFamHw) { using T = Bfp<16>; __VA_ARGS__; } break;     \
+    default: if constexpr (FAM != kFamHw) { using T = Bfp<25>; __VA_ARGS__; } break;      \
+    }
+
 // Runs the statements with T bound to the element type of storage format f.
 #define HMVM_FMT_SWITCH(f, ...)                                      \
     switch (f) {                                                     \

@@ -39,13 +39,13 @@ __global__ __launch_bounds__(kThr) void k_xtrans(const XT* __restrict__ X, int64
 }
 
 // one workgroup per V segment (x slice 64 x Rp in dynamic LDS)
-template <int NTM>
+template <int NTM, int FAM>
 __global__ __launch_bounds__(kThr) void k_vfwd_mm(DevPlan P, const double* __restrict__ XTb, int R,
                                                   double* __restrict__ Pb) {
     extern __shared__ double xs[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const VItem sg = P.vitems[blockIdx.x];
-    vseg_mm_dispatch<NTM>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
+    vseg_mm_dispatch<NTM, FAM>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
 }
 
 __global__ __launch_bounds__(256) void k_psum1_mm(DevPlan D, const double* __restrict__ Pb, double* __restrict__ tmp,
@@ -60,22 +60,22 @@ __global__ __launch_bounds__(256) void k_psum2_mm(DevPlan D, const double* __res
     if (g < D.ntent * R) psum2_item(D, tmp, t, R, alpha, g);
 }
 
-template <int NTM, typename YT>
+template <int NTM, typename YT, int FAM>
 __global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const double* __restrict__ XTb,
                                                   const double* __restrict__ t, YT* __restrict__ Y, int64_t ldy,
                                                   int R) {
     extern __shared__ double zbuf[];               // 64 x kMmStride, sized at launch
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const RowPanel Rw = P.rp[blockIdx.x];
-    rpanel_mm<NTM>(P, Rw, alpha, XTb, t, Y, ldy, R, zbuf, tid, wave, lane);
+    rpanel_mm<NTM, YT, FAM>(P, Rw, alpha, XTb, t, Y, ldy, R, zbuf, tid, wave, lane);
 }
 
 } // namespace
 
-int launch_apply_mrhs(const DevPlan& D, int64_t max_ws_part, double alpha, const void* x, int64_t ldx, void* y,
-                      int64_t ldy, int R, int vdt, double* ws, hipStream_t st) {
+int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, int64_t ldx, void* y, int64_t ldy,
+                      int R, int vdt, double* ws, hipStream_t st) {
     if (R < 2 || R > kMaxRp) return 1;
-    (void)max_ws_part;
+    if (fam != kFamHw && fam != kFamBfp) return 1;
     const int64_t pad = 32;
     // workspace: partials | stage-1 sums | t | XT (ncols x R)
     double* part = ws;
@@ -92,16 +92,27 @@ int launch_apply_mrhs(const DevPlan& D, int64_t max_ws_part, double alpha, const
             k_xtrans<float><<<blocks(D.ncols, 64), kThr, lds, st>>>(static_cast<const float*>(x), ldx, D.ncols, R, xt);
     }
     // accumulators sized for exactly the ceil(R / 16) tiles of this call
+    // one instantiation per format family: a plan's family carries only its
+    // own formats' decode paths (and registers)
     auto vpass = [&](auto ntm) {
         constexpr int NTM = decltype(ntm)::value;
-        k_vfwd_mm<NTM><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
+        if (fam == kFamHw) k_vfwd_mm<NTM, kFamHw><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
+        else k_vfwd_mm<NTM, kFamBfp><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
     };
     auto rpass = [&](auto ntm) {
         constexpr int NTM = decltype(ntm)::value;
-        if (vdt == 0)
-            k_ufwd_mm<NTM, double><<<unsigned(D.nrp), kThr, lds, st>>>(D, alpha, xt, t, static_cast<double*>(y), ldy, R);
-        else
-            k_ufwd_mm<NTM, float><<<unsigned(D.nrp), kThr, lds, st>>>(D, alpha, xt, t, static_cast<float*>(y), ldy, R);
+        auto go = [&](auto ytag, auto famtag) {
+            using YT = decltype(ytag);
+            constexpr int FAM = decltype(famtag)::value;
+            k_ufwd_mm<NTM, YT, FAM><<<unsigned(D.nrp), kThr, lds, st>>>(D, alpha, xt, t, static_cast<YT*>(y), ldy, R);
+        };
+        if (vdt == 0) {
+            if (fam == kFamHw) go(double(0), std::integral_constant<int, kFamHw>());
+            else go(double(0), std::integral_constant<int, kFamBfp>());
+        } else {
+            if (fam == kFamHw) go(float(0), std::integral_constant<int, kFamHw>());
+            else go(float(0), std::integral_constant<int, kFamBfp>());
+        }
     };
     auto by_tiles = [&](auto pass) {
         switch (Rp / 16) {

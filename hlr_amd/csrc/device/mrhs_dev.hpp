@@ -167,10 +167,10 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
 
 // NTM: tiles of 16 right-hand sides the accumulators are sized for
 // (compile time); the call handles ceil(R / 16) <= NTM of them
-template <int NTM>
+template <int NTM, int FAM = kFamAll>
 __device__ inline void vseg_mm_dispatch(const DevPlan& P, const VItem& sg, const double* __restrict__ XTb,
                                         int R, double* __restrict__ Pb, double* xs, int tid, int wave, int lane) {
-    HMVM_FMT_SWITCH(sg.fmt, vseg_mm_wg<NTM, T>(P, sg, XTb, R, Pb, xs, tid, wave, lane))
+    HMVM_FMT_SWITCH_FAM(FAM, sg.fmt, vseg_mm_wg<NTM, T>(P, sg, XTb, R, Pb, xs, tid, wave, lane))
 }
 
 // ---------------------------------------------------------------------------
@@ -282,7 +282,7 @@ template <> struct RAcc<double> {
 };
 
 // whole row panel, Y(:, 0..R) += R_p Z; all kThr threads call it
-template <int NTM, typename YT>
+template <int NTM, typename YT, int FAM = kFamAll>
 __device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double alpha, const double* __restrict__ XTb,
                                  const double* __restrict__ t, YT* __restrict__ Y, int64_t ldy, int R,
                                  double* zbuf, int tid, int wave, int lane) {
@@ -293,8 +293,8 @@ __device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double al
     for (int i = 0; i < NTM; ++i) { acc64[i] = d4{0, 0, 0, 0}; acc32[i] = f4{0, 0, 0, 0}; }
     for (int ci = 0; ci < Rw.nchunks; ++ci) {
         const RChunk Ch = P.rchunks[Rw.chunk0 + ci];
-        HMVM_FMT_SWITCH(Ch.fmt, rchunk_mm<NTM, T>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf,
-                                                   RAcc<T>::pick(acc64, acc32), tid, wave, lane))
+        HMVM_FMT_SWITCH_FAM(FAM, Ch.fmt, rchunk_mm<NTM, T>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf,
+                                                            RAcc<T>::pick(acc64, acc32), tid, wave, lane))
     }
     // bring the FP64 tiles into the FP32 element order (row 4 kk + e), then
     // one writer per output element: row 16 wave + 4 (lane / 16) + e

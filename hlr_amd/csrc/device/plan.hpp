@@ -40,6 +40,12 @@ enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, B8 = 4, B16 = 5, B25 =
 constexpr int kFirstBfp = B8;
 constexpr int kNumBfp = NFMT - kFirstBfp;
 
+// Format families: a plan holds formats of one family only (the plan
+// builder enforces it), so a kernel instantiated for that family carries the
+// decode paths, and the registers, of its own formats alone.  FP64 and FP32
+// belong to both.
+enum FmtFamily : int { kFamAll = 0, kFamHw = 1, kFamBfp = 2 };
+
 constexpr int kPanel = 64;          // rows per panel = lanes per wave
 constexpr int kWaves = 4;           // waves per workgroup
 constexpr int kChunkGroups = 64 * kWaves;   // column groups per R chunk (one per lane of the workgroup)

@@ -769,6 +769,16 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
 
     // ---- statistics ----
     for (int f = 0; f < NFMT; ++f) L.bytes_by_format[f] = fmt_store_bytes(f, L.nvals[f]);
+    {
+        bool hw = false, bfp = false;
+        for (int f = 0; f < NFMT; ++f) {
+            if (L.nvals[f] == 0) continue;
+            hw |= (f == BF16 || f == FP8);
+            bfp |= fmt_is_bfp(f);
+        }
+        if (hw && bfp) return HMVM_ERR_INVALID_ARG;     // e.g. BF16 forced beside block floating point dense leaves
+        L.family = bfp ? kFamBfp : kFamHw;
+    }
     // bytes per stored value (16 bytes per row of W values)
     auto vbytes = [](int f) { return 16.0 / double(fmt_width(f)); };
     double fl = 0, ds = 0, br = 0, bv = 0;

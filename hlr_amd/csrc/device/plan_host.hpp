@@ -65,6 +65,7 @@ struct HostLayout {
     int64_t cols_by_format[NFMT] = {};
     double flops = 0, datasize = 0, bytes_rpass = 0, bytes_vpass = 0;
     int32_t max_vitem_rows = 0;
+    int32_t family = kFamHw;       // format family of every stored column
     int64_t max_zbytes = 0;        // largest per-row-panel z block (bytes)
 
     int64_t arena_elems(int f) const { return nvals[f]; }

@@ -124,7 +124,9 @@ typedef struct hmvm_plan_opts {
     int32_t check_finite; /* reject NaN/Inf factors at creation (default 1)       */
     int32_t force_format; /* 0: formats from `precision`; 1 + HMVM_FMT_*: store
                              every low-rank column in that format (experiments
-                             and format tests)                                    */
+                             and format tests).  A plan may not hold BF16 or
+                             FP8 columns beside block floating point ones
+                             (HMVM_ERR_INVALID_ARG)                               */
 } hmvm_plan_opts;
 
 typedef struct hmvm_plan_info {

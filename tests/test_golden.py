@@ -58,7 +58,8 @@ def test_oracle_tlr_exact():
         assert np.array_equal(yt, g["yt"])
 
 
-@pytest.mark.parametrize("prec,force", [("fp64", None), ("fp32", None), ("valr", "bf16"), ("valr", "fp8_e4m3")])
+@pytest.mark.parametrize("prec,force", [("fp64", None), ("fp32", None), ("valr", "bf16"), ("valr", "fp8_e4m3"),
+                                        ("bfp", "bfp8"), ("bfp", "bfp16"), ("bfp", "bfp25")])
 def test_layout_holds_integers_exactly(g, prec, force):
     """Small integers are exact in every storage format (FP8-E4M3 after the
     power-of-two column scale), so unpacking the stored plan must give the

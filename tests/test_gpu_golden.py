@@ -1,6 +1,6 @@
 """Exact-arithmetic fixtures on the GPU: the plans in every storage format
-(FP64; FP32; every low-rank column forced to BF16 or to FP8-E4M3 with its
-power-of-two scale) must return the fixture's integer results bit for bit,
+(FP64; FP32; every low-rank column forced to BF16, to FP8-E4M3 with its
+power-of-two scale, or to 8-, 16- or 25-bit block floating point) must return the fixture's integer results bit for bit,
 for the normal and transposed products, three right-hand sides on the
 matrix cores, FP32 vectors, the one-level TLR entry point and a batch.
 Every partial sum on these inputs is an integer below 2^24 (checked on the
@@ -18,7 +18,8 @@ from tests.golden_data import CASES, load
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = [("fp64", None), ("fp32", None), ("valr", "bf16"), ("valr", "fp8_e4m3")]
+CONFIGS = [("fp64", None), ("fp32", None), ("valr", "bf16"), ("valr", "fp8_e4m3"), ("bfp", "bfp8"), ("bfp", "bfp16"),
+           ("bfp", "bfp25")]
 
 
 def _plan(g, prec, force):
@@ -55,7 +56,7 @@ def test_exact(cuda, case, prec, force):
         assert np.array_equal(yt.cpu().numpy(), g["yt"])
 
 
-@pytest.mark.parametrize("prec,force", [("fp64", None), ("valr", "fp8_e4m3")])
+@pytest.mark.parametrize("prec,force", [("fp64", None), ("valr", "fp8_e4m3"), ("bfp", "bfp8")])
 def test_exact_batch(cuda, prec, force):
     import torch
     gs = [load(c) for c in CASES]

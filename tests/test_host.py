@@ -170,6 +170,9 @@ def test_bfp_force_format_and_errors(sphere):
         assert c[f] == sum(c) > 0
     with pytest.raises(RuntimeError):
         Plan.from_hmatrix(sphere, precision="bfp", eps=0.0, device=-1)
+    # BF16 / FP8 columns may not sit beside block floating point dense leaves
+    with pytest.raises(RuntimeError):
+        Plan.from_hmatrix(sphere, precision="bfp", eps=1e-6, device=-1, force_format="bf16")
 
 
 def test_dense_and_datasize_valr(sphere):
