@@ -4,7 +4,7 @@
 // matrix/lrmatrix.hh:625-679 and matrix/dense_matrix.hh:459-481):
 //   k_vfwd    P[e, q] = sum_{i in q} V(i, e) x_i     one wave per pack of <= 64
 //                                                    column groups (lane = group)
-//   k_psum*   t_e = a S_e sum_q P[e, q]              two-stage segmented sums
+//   k_psum    t_e = a S_e sum_q P[e, q]              lanes per entry, wave tree
 //   k_ufwd    y_p += sum_c R_p(:, c) z_c             one workgroup per row
 //             z_c = a x_j (dense) | t_e (low-rank)   panel, lane = row
 // Transposed y += a A^T x (arith/blas.hh:910-948):
@@ -53,36 +53,38 @@ __global__ __launch_bounds__(kThreads, 4) void k_vfwd(DevPlan P, const XT* __res
     HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(P, pk, x, Pb, xs[wave], lane))
 }
 
-// stage 1: tmp[s] = sum of kSegLen consecutive partials of entry se[s]
-__global__ __launch_bounds__(256) void k_psum1(const int32_t* __restrict__ se, const int32_t* __restrict__ sr0,
-                                               const int32_t* __restrict__ off, const int32_t* __restrict__ cnt,
-                                               const int32_t* __restrict__ str, const double* __restrict__ Pb,
-                                               double* __restrict__ tmp, int64_t ns) {
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
-    const int32_t e = se[s], r0 = sr0[s];
-    const int32_t o = off[e], k = str[e];
-    const int32_t r1 = min(r0 + kSegLen, cnt[e]);
-    double v[kSegLen];
+// t_e = alpha * S_e * sum of the entry's partials, in one kernel: the lanes
+// of entry e are adjacent in one wave; lane i sums segments i, i + L, ... of
+// kSegLen partials (each in order), then a fixed tree over the entry's lanes
+// (lane i takes lane i + s when i % 2s == 0) leaves the sum in lane 0.
+__global__ __launch_bounds__(256) void k_psum(const int32_t* __restrict__ le, const int32_t* __restrict__ lil,
+                                              const int32_t* __restrict__ off, const int32_t* __restrict__ cnt,
+                                              const int32_t* __restrict__ str, const double* __restrict__ tmul,
+                                              const double* __restrict__ Pb, double* __restrict__ t, int64_t nl,
+                                              double alpha) {
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;   // nl is a multiple of 64
+    int32_t e = -1, il = 0;
+    if (g < nl) { e = le[g]; il = lil[g]; }
+    const int i = il & 255, L = il >> 8;
+    double v = 0;
+    if (e >= 0) {
+        const int32_t o = off[e], c = cnt[e], k = str[e];
+        for (int32_t r0 = i * kSegLen; r0 < c; r0 += L * kSegLen) {
+            double a[kSegLen];
 #pragma unroll
-    for (int r = 0; r < kSegLen; ++r) v[r] = (r0 + r < r1) ? Pb[o + int64_t(r0 + r) * k] : 0.0;
-    double a = 0;
+            for (int r = 0; r < kSegLen; ++r) a[r] = (r0 + r < c) ? Pb[o + int64_t(r0 + r) * k] : 0.0;
+            double s = 0;
 #pragma unroll
-    for (int r = 0; r < kSegLen; ++r) a += v[r];
-    tmp[s] = a;
-}
-
-// stage 2: t_e = alpha * S_e * sum_j tmp[pos0_e + j * stride_e]
-__global__ __launch_bounds__(256) void k_psum2(const int32_t* __restrict__ pos0, const int32_t* __restrict__ nseg,
-                                               const int32_t* __restrict__ str, const double* __restrict__ tmul,
-                                               const double* __restrict__ tmp, double* __restrict__ t, int64_t ne,
-                                               double alpha) {
-    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    const int32_t p = pos0[e], n = nseg[e], k = str[e];
-    double a = 0;
-    for (int32_t j = 0; j < n; ++j) a += tmp[p + int64_t(j) * k];
-    t[e] = alpha * tmul[e] * a;
+            for (int r = 0; r < kSegLen; ++r) s += a[r];
+            v += s;
+        }
+    }
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const double w = __shfl_down(v, s, 64);
+        if ((i & (2 * s - 1)) == 0 && i + s < L) v += w;
+    }
+    if (e >= 0 && i == 0) t[e] = alpha * tmul[e] * v;
 }
 
 // ---------------------------------------------------------------------------
@@ -410,7 +412,13 @@ int upload_plan(hmvm_plan* p) {
     UP(qs_r0, L.qs_r0);
     UP(qs_pos0, L.qs_pos0);
     UP(qs_nseg, L.qs_nseg);
+    UP(pl_e, L.pl_e);
+    UP(pl_il, L.pl_il);
+    UP(ql_e, L.ql_e);
+    UP(ql_il, L.ql_il);
 #undef UP
+    D.npl = int64_t(L.pl_e.size());
+    D.nql = int64_t(L.ql_e.size());
     std::vector<RtItem> items;
     for (int f = 0; f < NFMT; ++f) {
         p->rtitem_start[f] = int64_t(items.size());
@@ -466,17 +474,13 @@ int64_t ws_doubles(const hmvm_plan* p, int op) {
     return pad32(a) + pad32(sg) + pad32(L.ntent) + 32;
 }
 
-// t = alpha S sum(partials) in two stages
-void launch_psum(const DevPlan& D, bool trans, const double* part, double* tmp, double* t, double alpha,
-                 hipStream_t st) {
-    const int64_t ns = trans ? D.nqs : D.nps;
-    if (D.ntent == 0) return;
-    if (ns > 0)
-        k_psum1<<<blocks_for(ns, 256), 256, 0, st>>>(trans ? D.qs_e : D.ps_e, trans ? D.qs_r0 : D.ps_r0,
-                                                     trans ? D.qt_off : D.pt_off, trans ? D.qt_cnt : D.pt_cnt,
-                                                     D.tstride, part, tmp, ns);
-    k_psum2<<<blocks_for(D.ntent, 256), 256, 0, st>>>(trans ? D.qs_pos0 : D.ps_pos0, trans ? D.qs_nseg : D.ps_nseg,
-                                                      D.tstride, D.tmul, tmp, t, D.ntent, alpha);
+// t = alpha S sum(partials), one kernel (k_psum)
+void launch_psum(const DevPlan& D, bool trans, double* t, double alpha, const double* part, hipStream_t st) {
+    const int64_t nl = trans ? D.nql : D.npl;
+    if (D.ntent == 0 || nl == 0) return;
+    k_psum<<<blocks_for(nl, 256), 256, 0, st>>>(trans ? D.ql_e : D.pl_e, trans ? D.ql_il : D.pl_il,
+                                                trans ? D.qt_off : D.pt_off, trans ? D.qt_cnt : D.pt_cnt, D.tstride,
+                                                D.tmul, part, t, nl, alpha);
 }
 
 template <typename XT, typename YT>
@@ -497,7 +501,7 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
                 k_vfwd<XT, decltype(fam)::value><<<blocks_for(D.nvpacks, kWaves), kThreads, 0, st>>>(D, x, part);
             });
         if (ev) (void)hipEventRecord(ev[1], st);
-        launch_psum(D, false, part, tmp, t, alpha, st);
+        launch_psum(D, false, t, alpha, part, st);
         if (ev) (void)hipEventRecord(ev[2], st);
         if (D.nrp > 0) {
             // chunk staging needs kChunkGroups * 16 floats (FP8) or * 2 doubles (FP64): 16 KB
@@ -521,7 +525,7 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
             HMVM_FMT_SWITCH(f, k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part))
         }
         if (ev) (void)hipEventRecord(ev[1], st);
-        launch_psum(D, true, part, tmp, t, alpha, st);
+        launch_psum(D, true, t, alpha, part, st);
         if (ev) (void)hipEventRecord(ev[2], st);
         if (D.ncp > 0)
             by_family(p->L.family, [&](auto fam) {
@@ -773,15 +777,13 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
     const HostLayout& L = p->L;
     const DevPlan& D = p->dp;
     const int64_t np = pad32(L.psize), nq = pad32(L.qsize), ne = pad32(L.ntent) + 32;
-    const int64_t nsg = pad32(std::max<int64_t>(D.nps, D.nqs));
-    const int64_t bytes = (np + nq + 2 * ne + nsg) * int64_t(sizeof(double));
+    const int64_t bytes = (np + nq + 2 * ne) * int64_t(sizeof(double));
     void* ws = nullptr;
     if (hipMallocAsync(&ws, size_t(bytes), st) != hipSuccess) return HMVM_ERR_ALLOC;
     double* Pb = static_cast<double*>(ws);
     double* Qb = Pb + np;
     double* t = Qb + nq;
     double* s = t + ne;
-    double* tmp = s + ne;
     auto run = [&](auto xt_tag) -> int {
         using XT = decltype(xt_tag);
         const XT* xx = static_cast<const XT*>(x);
@@ -798,8 +800,8 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
             const unsigned nb = blocks_for(n, kWaves);
             HMVM_FMT_SWITCH(f, k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb))
         }
-        launch_psum(D, false, Pb, tmp, t, 1.0, st);
-        launch_psum(D, true, Qb, tmp, s, 1.0, st);
+        launch_psum(D, false, t, 1.0, Pb, st);
+        launch_psum(D, true, s, 1.0, Qb, st);
         if (gx && D.ncp > 0)
             by_family(p->L.family, [&](auto fam) {
                 k_vtrn<XT, decltype(fam)::value><<<unsigned(D.ncp), kThreads, 0, st>>>(D, 1.0, s, Qb,

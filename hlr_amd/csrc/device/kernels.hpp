@@ -43,6 +43,11 @@ struct DevPlan {
     const int32_t* qs_r0;
     const int32_t* qs_pos0;
     const int32_t* qs_nseg;
+    const int32_t* pl_e;      // one-kernel sums: lanes per t-entry (see the plan builder)
+    const int32_t* pl_il;
+    const int32_t* ql_e;
+    const int32_t* ql_il;
+    int64_t npl, nql;
     int64_t nvpacks, nps, nqs;
     const int32_t* qd_off;
     const int32_t* qd_cnt;

@@ -47,6 +47,12 @@ struct HostLayout {
     // pos0[e] + j * tstride[e], j < nseg[e]
     std::vector<int32_t> ps_e, ps_r0, ps_pos0, ps_nseg;   // forward (P)
     std::vector<int32_t> qs_e, qs_r0, qs_pos0, qs_nseg;   // transposed (Q)
+    // one-kernel sums for one right-hand side: lanes of t-entry e (at most 64,
+    // adjacent, inside one wave); lane i sums segments i, i + L, ... of
+    // kSegLen partials and the wave adds up the entry's lanes.  Per lane: e
+    // (-1: padding) and i | L << 8
+    std::vector<int32_t> pl_e, pl_il;   // forward (P)
+    std::vector<int32_t> ql_e, ql_il;   // transposed (Q)
     std::vector<int32_t> qd_off, qd_cnt;   // transposed dense partials per column
     int64_t psize = 0, qsize = 0;
 
