@@ -57,18 +57,18 @@ __global__ __launch_bounds__(kThreads, 4) void k_vfwd(DevPlan P, const XT* __res
 // of entry e are adjacent in one wave; lane i sums segments i, i + L, ... of
 // kSegLen partials (each in order), then a fixed tree over the entry's lanes
 // (lane i takes lane i + s when i % 2s == 0) leaves the sum in lane 0.
-__global__ __launch_bounds__(256) void k_psum(const int32_t* __restrict__ le, const int32_t* __restrict__ lil,
-                                              const int32_t* __restrict__ off, const int32_t* __restrict__ cnt,
-                                              const int32_t* __restrict__ str, const double* __restrict__ tmul,
-                                              const double* __restrict__ Pb, double* __restrict__ t, int64_t nl,
-                                              double alpha) {
+__global__ __launch_bounds__(256) void k_psum(const SumLane* __restrict__ lanes, const int32_t* __restrict__ lil,
+                                              const double* __restrict__ tmul, const double* __restrict__ Pb,
+                                              double* __restrict__ t, int64_t nl, double alpha) {
     const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;   // nl is a multiple of 64
-    int32_t e = -1, il = 0;
-    if (g < nl) { e = le[g]; il = lil[g]; }
+    SumLane ln{0, 0, 0, -1};
+    int32_t il = 0;
+    if (g < nl) { ln = lanes[g]; il = lil[g]; }
+    const int32_t e = ln.e;
     const int i = il & 255, L = il >> 8;
     double v = 0;
     if (e >= 0) {
-        const int32_t o = off[e], c = cnt[e], k = str[e];
+        const int32_t o = ln.off, c = ln.cnt, k = ln.stride;
         for (int32_t r0 = i * kSegLen; r0 < c; r0 += L * kSegLen) {
             double a[kSegLen];
 #pragma unroll
@@ -412,13 +412,13 @@ int upload_plan(hmvm_plan* p) {
     UP(qs_r0, L.qs_r0);
     UP(qs_pos0, L.qs_pos0);
     UP(qs_nseg, L.qs_nseg);
-    UP(pl_e, L.pl_e);
+    UP(pl, L.pl);
     UP(pl_il, L.pl_il);
-    UP(ql_e, L.ql_e);
+    UP(ql, L.ql);
     UP(ql_il, L.ql_il);
 #undef UP
-    D.npl = int64_t(L.pl_e.size());
-    D.nql = int64_t(L.ql_e.size());
+    D.npl = int64_t(L.pl.size());
+    D.nql = int64_t(L.ql.size());
     std::vector<RtItem> items;
     for (int f = 0; f < NFMT; ++f) {
         p->rtitem_start[f] = int64_t(items.size());
@@ -478,9 +478,8 @@ int64_t ws_doubles(const hmvm_plan* p, int op) {
 void launch_psum(const DevPlan& D, bool trans, double* t, double alpha, const double* part, hipStream_t st) {
     const int64_t nl = trans ? D.nql : D.npl;
     if (D.ntent == 0 || nl == 0) return;
-    k_psum<<<blocks_for(nl, 256), 256, 0, st>>>(trans ? D.ql_e : D.pl_e, trans ? D.ql_il : D.pl_il,
-                                                trans ? D.qt_off : D.pt_off, trans ? D.qt_cnt : D.pt_cnt, D.tstride,
-                                                D.tmul, part, t, nl, alpha);
+    k_psum<<<blocks_for(nl, 256), 256, 0, st>>>(trans ? D.ql : D.pl, trans ? D.ql_il : D.pl_il, D.tmul, part, t, nl,
+                                                alpha);
 }
 
 template <typename XT, typename YT>

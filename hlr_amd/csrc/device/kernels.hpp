@@ -43,9 +43,9 @@ struct DevPlan {
     const int32_t* qs_r0;
     const int32_t* qs_pos0;
     const int32_t* qs_nseg;
-    const int32_t* pl_e;      // one-kernel sums: lanes per t-entry (see the plan builder)
+    const SumLane* pl;        // one-kernel sums: lanes per t-entry (see the plan builder)
     const int32_t* pl_il;
-    const int32_t* ql_e;
+    const SumLane* ql;
     const int32_t* ql_il;
     int64_t npl, nql;
     int64_t nvpacks, nps, nqs;

@@ -110,6 +110,13 @@ struct VItem {          // V segment: up to kItemGroups groups of one format of 
     int64_t data_off;   // element offset of the m x (ng*W) row-major block in arena[fmt]
 };
 
+// one lane of the single right-hand-side partial sums: entry e (-1:
+// padding) with its partials at off + r * stride, r < cnt; lane i of the
+// entry's L lanes (il = i | L << 8)
+struct SumLane {
+    int32_t off, cnt, stride, e;
+};
+
 // transposed / gradient R work item: groups [g0, min(g0 + kRtSpan, ngroups)) of a chunk
 constexpr int32_t kRtSpan = 32;
 struct RtItem {

@@ -466,19 +466,24 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         segs(true, L.ps_e, L.ps_r0, L.ps_pos0, L.ps_nseg);
         segs(false, L.qs_e, L.qs_r0, L.qs_pos0, L.qs_nseg);
         // lanes of the one-kernel sums: an entry's lanes never straddle a wave
-        auto lanes = [&](const std::vector<int32_t>& cnt, std::vector<int32_t>& le, std::vector<int32_t>& lil) {
+        auto lanes = [&](const std::vector<int32_t>& off, const std::vector<int32_t>& cnt, std::vector<SumLane>& le,
+                         std::vector<int32_t>& lil) {
             le.clear(); lil.clear();
+            const SumLane pad{0, 0, 0, -1};
             for (int64_t e = 0; e < ne; ++e) {
                 const int64_t nseg = (int64_t(cnt[e]) + kSegLen - 1) / kSegLen;
                 const int32_t Le = int32_t(std::min<int64_t>(std::max<int64_t>(nseg, 1), 64));
                 if (int64_t(le.size() % 64) + Le > 64)
-                    while (le.size() % 64) { le.push_back(-1); lil.push_back(0); }
-                for (int32_t i = 0; i < Le; ++i) { le.push_back(int32_t(e)); lil.push_back(i | (Le << 8)); }
+                    while (le.size() % 64) { le.push_back(pad); lil.push_back(0); }
+                for (int32_t i = 0; i < Le; ++i) {
+                    le.push_back({off[e], cnt[e], L.tstride[e], int32_t(e)});
+                    lil.push_back(i | (Le << 8));
+                }
             }
-            while (le.size() % 64) { le.push_back(-1); lil.push_back(0); }
+            while (le.size() % 64) { le.push_back(pad); lil.push_back(0); }
         };
-        lanes(L.pt_cnt, L.pl_e, L.pl_il);
-        lanes(L.qt_cnt, L.ql_e, L.ql_il);
+        lanes(L.pt_off, L.pt_cnt, L.pl, L.pl_il);
+        lanes(L.qt_off, L.qt_cnt, L.ql, L.ql_il);
         // dense transposed partials: per global column, one per (dense block, row panel)
         L.qd_cnt.assign(ncols, 0);
         L.qd_off.assign(ncols, 0);

@@ -51,8 +51,8 @@ struct HostLayout {
     // adjacent, inside one wave); lane i sums segments i, i + L, ... of
     // kSegLen partials and the wave adds up the entry's lanes.  Per lane: e
     // (-1: padding) and i | L << 8
-    std::vector<int32_t> pl_e, pl_il;   // forward (P)
-    std::vector<int32_t> ql_e, ql_il;   // transposed (Q)
+    std::vector<SumLane> pl;   std::vector<int32_t> pl_il;   // forward (P)
+    std::vector<SumLane> ql;   std::vector<int32_t> ql_il;   // transposed (Q)
     std::vector<int32_t> qd_off, qd_cnt;   // transposed dense partials per column
     int64_t psize = 0, qsize = 0;
 
