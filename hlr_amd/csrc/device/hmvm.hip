@@ -194,6 +194,69 @@ __global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const RtItem* __re
     }
 }
 
+// Transposed R pass for the FP32-accumulated formats (W >= 4), through LDS
+// instead of cross-lane reductions: a step takes GS groups (GS W <= 32
+// columns); every lane (row i) writes its products R(i, c) x_i into the
+// wave's tile [row][col] (row stride kTileStride floats, 16-byte aligned),
+// then lane L sums rows L / 32, L / 32 + 2, ... of column L % 32 in order and
+// one shuffle adds the two halves; lane c < GS W writes Q for column c.
+// The wave's 64 x 36 floats (9 KB) leave room for 4 workgroups per CU.
+constexpr int kTileCols = 32;
+constexpr int kTileStride = kTileCols + 4;
+
+template <typename T, typename XT>
+__global__ __launch_bounds__(kThreads) void k_rtrn_lds(DevPlan P, const RtItem* __restrict__ items, int64_t n,
+                                                       const XT* __restrict__ x, double* __restrict__ Q) {
+    using E = Elem<T>;
+    using A = typename E::Acc;
+    constexpr int W = E::W;
+    static_assert(sizeof(A) == 4 && W >= 4, "FP32-accumulated formats");
+    constexpr int GS = kTileCols / W;               // groups per step
+    __shared__ __align__(16) float tiles[kWaves][64 * kTileStride];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
+    if (idx >= n) return;                           // whole wave; no workgroup barrier follows
+    float* tile = tiles[wave];
+    const RtItem wi = items[idx];
+    const RChunk C = P.rchunks[wi.chunk];
+    const int gend = min(wi.g0 + kRtSpan, int(C.ngroups));
+    const RowPanel R = P.rp[C.panel];
+    const int m = R.m;
+    const bool on = lane < m;
+    const A xi = on ? A(x[R.row0 + lane]) : A(0);
+    const bytep base = elem_addr<T>(P.arena[E::F], C.data_off) + int64_t(on ? lane : 0) * 16;
+    const int64_t gs = int64_t(m) * 16;             // bytes per group
+    const int c = lane & 31, h = lane >> 5;
+#pragma unroll 1
+    for (int g0 = wi.g0; g0 < gend; g0 += GS) {
+        u32x4 raw[GS];
+#pragma unroll
+        for (int u = 0; u < GS; ++u)
+            raw[u] = (on && g0 + u < gend) ? raw16(base + int64_t(g0 + u) * gs) : u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+            A v[W];
+            E::decode(raw[u], v);                   // rows past m and groups past gend: zeros
+#pragma unroll
+            for (int w = 0; w < W; ++w) tile[lane * kTileStride + u * W + w] = v[w] * xi;
+        }
+        wave_sync();
+        float s = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) s += tile[(2 * r + h) * kTileStride + c];
+        s += __shfl_xor(s, 32, 64);
+        const int ncol = min(GS, gend - g0) * W;    // columns of this step
+        if (h == 0 && c < ncol) {
+            const int64_t gc = C.col0 + int64_t(g0) * W + c;
+            const int32_t d = P.rdest[gc];
+            double sv = double(s);
+            if (E::kScaled) sv *= double(P.zscale[gc]);
+            if (d >= 0) Q[d] = sv;
+        }
+        wave_sync();                                // the tile is rewritten by the next step
+    }
+}
+
 // ---------------------------------------------------------------------------
 // transposed: V pass, y_q += V_q s  (+ dense partials), workgroup per panel
 // ---------------------------------------------------------------------------
@@ -454,6 +517,16 @@ int upload_plan(hmvm_plan* p) {
     return HMVM_OK;
 }
 
+// transposed R pass of one format: cross-lane reductions for FP64, the LDS
+// tile for the FP32-accumulated formats
+template <typename T, typename XT>
+void launch_rtrn(const DevPlan& D, const RtItem* ids, int64_t n, const XT* x, double* Q, unsigned nb, hipStream_t st) {
+    if constexpr (std::is_same<T, double>::value)
+        k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, Q);
+    else
+        k_rtrn_lds<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, Q);
+}
+
 // calls f(std::integral_constant<int, FAM>) for the plan's format family
 template <typename F>
 void by_family(int fam, F&& f) {
@@ -521,7 +594,7 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
             if (n == 0) continue;
             const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
             const unsigned nb = blocks_for(n, kWaves);
-            HMVM_FMT_SWITCH(f, k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, part))
+            HMVM_FMT_SWITCH(f, launch_rtrn<T, XT>(D, ids, n, x, part, nb, st))
         }
         if (ev) (void)hipEventRecord(ev[1], st);
         launch_psum(D, true, t, alpha, part, st);
@@ -797,7 +870,7 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
             if (n == 0) continue;
             const RtItem* ids = p->rtitems_dev + p->rtitem_start[f];
             const unsigned nb = blocks_for(n, kWaves);
-            HMVM_FMT_SWITCH(f, k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, yb, Qb))
+            HMVM_FMT_SWITCH(f, launch_rtrn<T, XT>(D, ids, n, yb, Qb, nb, st))
         }
         launch_psum(D, false, t, 1.0, Pb, st);
         launch_psum(D, true, s, 1.0, Qb, st);
