@@ -8,8 +8,8 @@
 //   k_ufwd    y_p += sum_c R_p(:, c) z_c             one workgroup per row
 //             z_c = a x_j (dense) | t_e (low-rank)   panel, lane = row
 // Transposed y += a A^T x (arith/blas.hh:910-948):
-//   k_rtrn    Q[c] = sum_{i in p} R_p(i, c) x_i      wave per chunk, lane =
-//                                                    row, butterfly reduction
+//   k_rtrn_lds Q[c] = sum_{i in p} R_p(i, c) x_i     wave per work item, lane =
+//                                                    row, column sums via LDS
 //   k_psum*   s_e = a S_e sum_p Q[e, p]
 //   k_vtrn    y_q += V_q s + a sum Q_dense           workgroup per column panel
 // No atomics: every output element has exactly one writer and a fixed
@@ -128,81 +128,21 @@ __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, cons
 // transposed: R pass with column reduction, one wave per chunk
 // ---------------------------------------------------------------------------
 
-// Each step loads kRtLoads groups (raw 16-byte loads, all in flight
-// together), then reduces them across the lanes K columns at a time.  A
-// wave takes one work item: at most kRtSpan consecutive groups of a chunk
+// A wave takes one work item: at most kRtSpan consecutive groups of a chunk
 // (every column belongs to one item, so each Q entry has one writer).
-constexpr int kRtLoads = 8;
-
-template <typename T, typename XT>
-__global__ __launch_bounds__(kThreads) void k_rtrn(DevPlan P, const RtItem* __restrict__ items, int64_t n,
-                                                   const XT* __restrict__ x, double* __restrict__ Q) {
-    using E = Elem<T>;
-    using A = typename E::Acc;
-    constexpr int W = E::W;
-    // K columns reduced together (16 registers live; 8 for FP64 and for
-    // W = 5, 6, where 16 would leave a third of the slots empty); a
-    // reduction covers SG whole groups and pads the K - SG W slots left
-    // with zeros, a step NL = a multiple of SG groups
-    constexpr int K = (sizeof(A) == 8 || W == 5 || W == 6) ? 8 : 16;
-    constexpr int SG = K / W;                       // groups per reduction
-    constexpr int NL = (kRtLoads / SG) * SG;        // groups per step
-    constexpr int LPC = 64 / K;                     // lanes holding one column sum
-    static_assert(SG >= 1 && NL >= SG, "a reduction covers whole groups, a step whole reductions");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
-    if (idx >= n) return;
-    const RtItem wi = items[idx];
-    const RChunk C = P.rchunks[wi.chunk];
-    const int gend = min(wi.g0 + kRtSpan, int(C.ngroups));
-    const RowPanel R = P.rp[C.panel];
-    const int m = R.m;
-    const A xi = (lane < m) ? A(x[R.row0 + lane]) : A(0);
-    const bool on = lane < m;
-    const bytep base = elem_addr<T>(P.arena[E::F], C.data_off) + int64_t(on ? lane : 0) * 16;
-    const int64_t gs = int64_t(m) * 16;                 // bytes per group
-    const int ncol = C.ngroups * W;
-#pragma unroll 1
-    for (int g0 = wi.g0; g0 < gend; g0 += NL) {
-        u32x4 raw[NL];
-#pragma unroll
-        for (int u = 0; u < NL; ++u)
-            raw[u] = (g0 + u < gend) ? raw16(base + (g0 + u) * gs) : u32x4{0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int sb = 0; sb < NL / SG; ++sb) {
-            A v[K];
-#pragma unroll
-            for (int gg = 0; gg < SG; ++gg) {
-                A u[W];
-                E::decode(raw[sb * SG + gg], u);
-#pragma unroll
-                for (int w = 0; w < W; ++w) v[gg * W + w] = u[w] * xi;     // xi = 0 on rows past m
-            }
-#pragma unroll
-            for (int k = SG * W; k < K; ++k) v[k] = A(0);
-            const A s = xreduce<K>(v, lane);
-            const int slot = lane / LPC;
-            const int col = (g0 + sb * SG) * W + slot;
-            if ((lane % LPC) == 0 && slot < SG * W && col < min(ncol, gend * W)) {
-                const int64_t gc = C.col0 + col;
-                const int32_t d = P.rdest[gc];
-                double sv = double(s);
-                if (E::kScaled) sv *= double(P.zscale[gc]);
-                if (d >= 0) Q[d] = sv;
-            }
-        }
-    }
-}
-
-// Transposed R pass for the FP32-accumulated formats (W >= 4), through LDS
-// instead of cross-lane reductions: a step takes GS groups (GS W <= 32
-// columns); every lane (row i) writes its products R(i, c) x_i into the
-// wave's tile [row][col] (row stride kTileStride floats, 16-byte aligned),
-// then lane L sums rows L / 32, L / 32 + 2, ... of column L % 32 in order and
-// one shuffle adds the two halves; lane c < GS W writes Q for column c.
-// The wave's 64 x 36 floats (9 KB) leave room for 4 workgroups per CU.
-constexpr int kTileCols = 32;
-constexpr int kTileStride = kTileCols + 4;
+// Transposed R pass through LDS instead of cross-lane reductions: a step
+// takes GS groups (GS W <= TC columns); every lane (row i) writes its
+// products R(i, c) x_i into the wave's tile [row][col] (row stride TS
+// elements, 16-byte aligned), then lane L sums rows L / TC, L / TC + LPC,
+// ... of column L % TC in order and log2(LPC) shuffles add the LPC partial
+// sums; lane c < GS W writes Q for column c.  FP32 accumulation: TC = 32,
+// FP64: TC = 16; either way the wave's tile is 9 KB, which leaves room for
+// 4 workgroups per CU.
+template <typename A> struct RtTile;
+template <> struct RtTile<float> { static constexpr int TC = 32, TS = 36; };
+template <> struct RtTile<double> { static constexpr int TC = 16, TS = 18; };
+constexpr size_t kRtTileBytes = 64 * 36 * 4;
+static_assert(64 * RtTile<double>::TS * 8 == kRtTileBytes, "one tile size for both accumulation types");
 
 template <typename T, typename XT>
 __global__ __launch_bounds__(kThreads) void k_rtrn_lds(DevPlan P, const RtItem* __restrict__ items, int64_t n,
@@ -210,13 +150,14 @@ __global__ __launch_bounds__(kThreads) void k_rtrn_lds(DevPlan P, const RtItem* 
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
-    static_assert(sizeof(A) == 4 && W >= 4, "FP32-accumulated formats");
-    constexpr int GS = kTileCols / W;               // groups per step
-    __shared__ __align__(16) float tiles[kWaves][64 * kTileStride];
+    constexpr int TC = RtTile<A>::TC, TS = RtTile<A>::TS, LPC = 64 / TC;
+    constexpr int GS = TC / W;                      // groups per step
+    static_assert(GS >= 1, "a step holds at least one group");
+    __shared__ __align__(16) unsigned char tiles[kWaves][kRtTileBytes];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
     if (idx >= n) return;                           // whole wave; no workgroup barrier follows
-    float* tile = tiles[wave];
+    A* tile = reinterpret_cast<A*>(tiles[wave]);
     const RtItem wi = items[idx];
     const RChunk C = P.rchunks[wi.chunk];
     const int gend = min(wi.g0 + kRtSpan, int(C.ngroups));
@@ -226,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_rtrn_lds(DevPlan P, const RtItem* 
     const A xi = on ? A(x[R.row0 + lane]) : A(0);
     const bytep base = elem_addr<T>(P.arena[E::F], C.data_off) + int64_t(on ? lane : 0) * 16;
     const int64_t gs = int64_t(m) * 16;             // bytes per group
-    const int c = lane & 31, h = lane >> 5;
+    const int c = lane % TC, h = lane / TC;
 #pragma unroll 1
     for (int g0 = wi.g0; g0 < gend; g0 += GS) {
         u32x4 raw[GS];
@@ -238,13 +179,14 @@ __global__ __launch_bounds__(kThreads) void k_rtrn_lds(DevPlan P, const RtItem* 
             A v[W];
             E::decode(raw[u], v);                   // rows past m and groups past gend: zeros
 #pragma unroll
-            for (int w = 0; w < W; ++w) tile[lane * kTileStride + u * W + w] = v[w] * xi;
+            for (int w = 0; w < W; ++w) tile[lane * TS + u * W + w] = v[w] * xi;
         }
         wave_sync();
-        float s = 0.0f;
+        A s = A(0);
 #pragma unroll
-        for (int r = 0; r < 32; ++r) s += tile[(2 * r + h) * kTileStride + c];
-        s += __shfl_xor(s, 32, 64);
+        for (int r = 0; r < 64 / LPC; ++r) s += tile[(LPC * r + h) * TS + c];
+#pragma unroll
+        for (int o = TC; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
         const int ncol = min(GS, gend - g0) * W;    // columns of this step
         if (h == 0 && c < ncol) {
             const int64_t gc = C.col0 + int64_t(g0) * W + c;
@@ -517,14 +459,10 @@ int upload_plan(hmvm_plan* p) {
     return HMVM_OK;
 }
 
-// transposed R pass of one format: cross-lane reductions for FP64, the LDS
-// tile for the FP32-accumulated formats
+// transposed R pass of one format
 template <typename T, typename XT>
 void launch_rtrn(const DevPlan& D, const RtItem* ids, int64_t n, const XT* x, double* Q, unsigned nb, hipStream_t st) {
-    if constexpr (std::is_same<T, double>::value)
-        k_rtrn<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, Q);
-    else
-        k_rtrn_lds<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, Q);
+    k_rtrn_lds<T, XT><<<nb, kThreads, 0, st>>>(D, ids, n, x, Q);
 }
 
 // calls f(std::integral_constant<int, FAM>) for the plan's format family

@@ -193,22 +193,6 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// sum v[c] over the 64 lanes for c = 0..63; lane L returns the sum for c = L
-template <typename A>
-__device__ inline A butterfly64(A (&v)[64], int lane) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) {
-        const bool up = (lane & s) != 0;
-#pragma unroll
-        for (int j = 0; j < s; ++j) {
-            const A send = up ? v[j] : v[j + s];
-            const A keep = up ? v[j + s] : v[j];
-            v[j] = keep + __shfl_xor(send, s, 64);
-        }
-    }
-    return v[0];
-}
-
 // Sum v[k] over the 64 lanes for k = 0..K-1 (K a power of two <= 64):
 // log2(K) halving steps (each lane keeps one half and adds its partner's),
 // then log2(64 / K) plain steps.  Afterwards lane L holds the sum for
