@@ -219,14 +219,18 @@ __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VIte
     constexpr int W = E::W;
     const bool on = lane < it.ng;
     const int g = on ? lane : 0;
-    double sv[W];
+    // s values (times the column scales) in the accumulation type: a row's
+    // dot product over the lane's W columns is formed as in the forward
+    // passes (FP32 for the FP32-accumulated formats), the row sums across
+    // lanes in FP64
+    A sv[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const int64_t vc = C.vcol0[E::F] + int64_t(it.g0 + g) * W + w;
         const int32_t e = on ? P.vsrc[vc] : -1;
         double v = (e >= 0) ? s[e] : 0.0;
         if (E::kScaled && on) v *= double(P.vscale[vc]);
-        sv[w] = v;
+        sv[w] = A(v);
     }
     const int64_t ldb = int64_t(it.ng) * 16;            // bytes per row of the item
     const bytep base = elem_addr<T>(P.arena[E::F], it.data_off) + int64_t(g) * 16;
@@ -247,10 +251,10 @@ __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VIte
             for (int r = 0; r < kTRows; ++r) {
                 A v[W];
                 E::decode(raw[sb * kTRows + r], v);
-                double acc = 0.0;
+                A acc = A(0);
 #pragma unroll
-                for (int w = 0; w < W; ++w) acc += double(v[w]) * sv[w];
-                d[r] = acc;
+                for (int w = 0; w < W; ++w) acc += v[w] * sv[w];
+                d[r] = double(acc);
             }
             const double r = xreduce<kTRows>(d, lane);
             const int row = i0 + sb * kTRows + lane / (64 / kTRows);
