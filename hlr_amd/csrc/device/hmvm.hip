@@ -10,7 +10,7 @@
 // Transposed y += a A^T x (arith/blas.hh:910-948):
 //   k_rtrn_lds Q[c] = sum_{i in p} R_p(i, c) x_i     wave per work item, lane =
 //                                                    row, column sums via LDS
-//   k_psum*   s_e = a S_e sum_p Q[e, p]
+//   k_psum    s_e = a S_e sum_p Q[e, p]
 //   k_vtrn    y_q += V_q s + a sum Q_dense           workgroup per column panel
 // No atomics: every output element has exactly one writer and a fixed
 // summation order, so results are bitwise reproducible run to run.
