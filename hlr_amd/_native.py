@@ -114,6 +114,9 @@ HMVM_SYMBOLS = [
     ("hmvm_batch_tune", _i32, [_vp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_vp), C.POINTER(_i64), _vp,
                                C.POINTER(_i32)]),
     ("hmvm_spectral_norm", _i32, [_vp, C.c_double, _i64, _i32, _vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
+    ("hmvm_inv_error_2", _i32, [_vp, _vp, C.c_double, _i64, _vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
+    ("hmvm_linear_iteration", _i32, [_vp, _vp, _vp, _vp, _vp, C.c_double, _i64, _vp, C.POINTER(_i64),
+                                     C.POINTER(C.c_double)]),
     ("hmvm_error_string", C.c_char_p, [_i32]),
 ]
 

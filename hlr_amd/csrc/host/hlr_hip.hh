@@ -7,6 +7,8 @@
 //   hlr::hip::hip_arithmetic                   (tbb/arith.hh:1414-1510,
 //                                               arith/defaults.hh:20-104)
 //   hlr::hip::norm::spectral(arith, A, ...)    (arith/norm.hh:89-101)
+//   hlr::hip::norm::inv_error_2(arith, A, W)   (arith/norm.hh:115-137)
+//   hlr::hip::linear_iteration(A, W, b, x, ...) (programs/iterrefine.hh:20-80)
 // with y += alpha op(M) x, alpha == 0 a no-op, op in {apply_normal,
 // apply_conjugate, apply_transposed, apply_adjoint}, and errors thrown as
 // std::runtime_error (HLR_ASSERT/HLR_ERROR, utils/log.hh:40-67).  The matrix
@@ -157,17 +159,45 @@ struct hip_arithmetic {
 namespace norm {
 
 // spectral norm by power iteration on the device (arith/detail/norm.hh:448-527,
-// called as hlr::norm::spectral(arithmetic, A, atol, amax_it, squared),
-// arith/norm.hh:89-101)
-inline double spectral(const hip_arithmetic& arith, const hmatrix& A, const double atol = 0,
-                       const size_t amax_it = 0, const bool squared = true) {
+// called as hlr::norm::spectral(arithmetic, A, atol, amax_it, squared) with
+// the defaults of arith/norm.hh:89-101)
+inline double spectral(const hip_arithmetic& arith, const hmatrix& A, const double atol = 1e-3,
+                       const size_t amax_it = 50, const bool squared = true) {
     double lambda = 0;
     check(hmvm_spectral_norm(A.plan(), atol, int64_t(amax_it), squared ? 1 : 0, arith.stream, &lambda, nullptr),
           "hmvm_spectral_norm");
     return lambda;
 }
 
+// |I - A A_inv|_2 (arith/norm.hh:115-137): norm::spectral with its defaults
+// on the operator I - A A_inv, applied factor by factor
+inline double inv_error_2(const hip_arithmetic& arith, const hmatrix& A, const hmatrix& A_inv) {
+    double err = 0;
+    check(hmvm_inv_error_2(A.plan(), A_inv.plan(), 1e-3, 50, arith.stream, &err, nullptr), "hmvm_inv_error_2");
+    return err;
+}
+
 } // namespace norm
+
+// linear iteration x <- x - W (A x - b) (programs/iterrefine.hh:20-80) until
+// |sol - x| <= tol (or |A x - b| <= tol |b| without sol) or max_steps steps;
+// returns the number of steps, and the per-step |r|, |c|, |e| in history
+inline int64_t linear_iteration(const hmatrix& A, const hmatrix& W, const device_vector<double>& b,
+                                device_vector<double>& x, const device_vector<double>* sol = nullptr,
+                                const double tol = 1e-8, const int64_t max_steps = 100,
+                                std::vector<double>* history = nullptr, hipStream_t stream = nullptr) {
+    const size_t n = size_t(A.nrows());
+    if (A.ncols() != A.nrows() || W.nrows() != A.nrows() || W.ncols() != A.nrows() || b.length() != n ||
+        x.length() != n || (sol && sol->length() != n))
+        throw std::runtime_error("linear_iteration: sizes do not match");
+    if (history) history->assign(size_t(3 * std::max<int64_t>(max_steps, 1)), 0.0);
+    int64_t steps = 0;
+    check(hmvm_linear_iteration(A.plan(), W.plan(), b.data(), x.data(), sol ? sol->data() : nullptr, tol,
+                                max_steps, stream, &steps, history ? history->data() : nullptr),
+          "hmvm_linear_iteration");
+    if (history) history->resize(size_t(3 * steps));
+    return steps;
+}
 
 } // namespace hip
 } // namespace hlr

@@ -214,3 +214,41 @@ class Plan:
                                 ptrs if factors else None, _vec_dtype(x), _stream_handle(stream))
         N.check(rc, "hmvm_grad")
         return garena
+
+
+def _f64_vector(t, n: int, what: str):
+    import torch
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or t.dim() != 1 or t.numel() != n \
+            or not t.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous float64 device vector of {n} entries")
+    return C.c_void_p(t.data_ptr())
+
+
+def inv_error_2(A: Plan, W: Plan, atol: float = 0.0, max_it: int = 0, stream=None):
+    """|I - A W|_2 by power iteration on the device (the reference's
+    hlr::norm::inv_error_2, arith/norm.hh:115-137, which runs norm::spectral
+    on I - A A_inv): returns (error, iterations).  atol = 0: sqrt(eps);
+    max_it = 0: max(5, n / 10)."""
+    err = C.c_double()
+    its = C.c_int64()
+    N.check(N.hmvm().hmvm_inv_error_2(A.handle, W.handle, float(atol), int(max_it), _stream_handle(stream),
+                                      C.byref(err), C.byref(its)), "hmvm_inv_error_2")
+    return float(err.value), int(its.value)
+
+
+def linear_iteration(A: Plan, W: Plan, b, x, sol=None, tol: float = 0.0, max_steps: int = 0, stream=None):
+    """x <- x - W (A x - b) on the device until |sol - x| <= tol (sol given)
+    or |A x - b| <= tol |b|, or max_steps steps (the reference's
+    linear_iteration, programs/iterrefine.hh:20-80; tol = 0: 1e-8,
+    max_steps = 0: 100).  b, x, sol: float64 tensors on the plans' device;
+    x holds the start value and receives the result.  Returns (steps,
+    history), history[i] = (|r| before step i, |c|, |e| after it)."""
+    n = A.nrows
+    limit = int(max_steps) if max_steps else 100
+    hist = np.full(3 * max(limit, 1), np.nan)
+    steps = C.c_int64()
+    N.check(N.hmvm().hmvm_linear_iteration(A.handle, W.handle, _f64_vector(b, n, "b"), _f64_vector(x, n, "x"),
+                                           _f64_vector(sol, n, "sol") if sol is not None else None, float(tol),
+                                           int(max_steps), _stream_handle(stream), C.byref(steps),
+                                           hist.ctypes.data_as(C.POINTER(C.c_double))), "hmvm_linear_iteration")
+    return int(steps.value), hist[:3 * steps.value].reshape(-1, 3)

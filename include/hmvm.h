@@ -311,6 +311,40 @@ int hmvm_spectral_norm(const hmvm_plan* plan, double atol, int64_t amax_it,
                        int squared, void* stream, double* norm,
                        int64_t* iterations);
 
+/*
+ * 2-norm of I - A W by the squared power iteration of hmvm_spectral_norm
+ * (same start vector, stopping rules and defaults), replacing
+ * hlr::norm::inv_error_2(arithmetic, A, A_inv) (arith/norm.hh:115-137),
+ * which runs norm::spectral on the sum operator I - A A_inv.  A and W are
+ * n x n plans on one device (W an approximate inverse of A, in any storage
+ * precision); per step t = x - A (W x), y = t - W^H (A^H t).  Synchronises
+ * `stream`.
+ */
+int hmvm_inv_error_2(const hmvm_plan* A, const hmvm_plan* W, double atol,
+                     int64_t amax_it, void* stream, double* error,
+                     int64_t* iterations);
+
+/*
+ * Linear iteration x <- x - W (A x - b), replacing linear_iteration(A, W)
+ * (programs/iterrefine.hh:20-80), where W is the inverse of an H-LU
+ * factorisation in FP64 or FP32 (:124-187); here W is any n x n plan on
+ * A's device, e.g. an approximate inverse stored in mixed precision, while
+ * every residual A x - b is computed with A's plan.  b, x and sol are device
+ * vectors of n doubles; x holds the start value and receives the result.
+ * Per step: c = W r, x -= c, |e| = |sol - x|, r = A x - b, with r = A x - b
+ * for the start value (the reference program starts from r = b at x = 0).
+ * Stops when |e| <= tol with sol given, as the reference (tol == 0: 1e-8),
+ * or when |r| <= tol |b| with sol == NULL, or after max_steps steps
+ * (0: 100).  history (may be NULL, 3 * max_steps doubles) receives per step
+ * |r| before the step, |c| and |e| after it (NaN without sol), the three
+ * norms the reference prints.  *steps (may be NULL) receives the number of
+ * steps taken.  Synchronises `stream` once per step.
+ */
+int hmvm_linear_iteration(const hmvm_plan* A, const hmvm_plan* W,
+                          const double* b, double* x, const double* sol,
+                          double tol, int64_t max_steps, void* stream,
+                          int64_t* steps, double* history);
+
 const char* hmvm_error_string(int code);
 
 #ifdef __cplusplus

@@ -248,6 +248,30 @@ def test_apply_on_host_plan_rejected():
     assert N.error_string(3) == "leaf blocks overlap"
 
 
+def test_consumers_reject_bad_arguments():
+    """The consumers of the product (spectral norm, inversion error, linear
+    iteration) return HMVM_ERR_INVALID_ARG for missing plans or vectors,
+    negative limits or tolerances, and host-only plans, before any device
+    work."""
+    lib = N.hmvm()
+    p = Plan(_one_block(), np.ones(16), 4, 4, device=-1)
+    v = np.zeros(4)
+    vp = C.c_void_p(v.ctypes.data)
+    out = C.c_double()
+    steps = C.c_int64(-7)
+    assert lib.hmvm_spectral_norm(p.handle, 0.0, 0, 1, None, C.byref(out), None) == 1
+    assert lib.hmvm_spectral_norm(p.handle, 0.0, -1, 1, None, C.byref(out), None) == 1
+    assert lib.hmvm_spectral_norm(None, 0.0, 0, 1, None, C.byref(out), None) == 1
+    assert lib.hmvm_inv_error_2(p.handle, p.handle, 0.0, 0, None, C.byref(out), None) == 1
+    assert lib.hmvm_inv_error_2(p.handle, None, 0.0, 0, None, C.byref(out), None) == 1
+    assert lib.hmvm_inv_error_2(p.handle, p.handle, 0.0, 0, None, None, None) == 1
+    for args in [(p.handle, p.handle, vp, vp, None, 0.0, 0), (None, p.handle, vp, vp, None, 0.0, 0),
+                 (p.handle, p.handle, None, vp, None, 0.0, 0), (p.handle, p.handle, vp, None, None, 0.0, 0),
+                 (p.handle, p.handle, vp, vp, None, -1.0, 0), (p.handle, p.handle, vp, vp, None, 0.0, -1)]:
+        assert lib.hmvm_linear_iteration(*args, None, C.byref(steps), None) == 1, args
+    assert steps.value == -7
+
+
 @pytest.mark.parametrize("eps,exact", [(1e-4, False), (1e-6, False), (1e-8, True)])
 def test_dense_precision_follows_eps(sphere, eps, exact):
     """Dense leaves under the default rule: FP32 while ceil(-log2 eps) <= 23,
