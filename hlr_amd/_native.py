@@ -18,6 +18,7 @@ ERRORS = {
     1: "HMVM_ERR_INVALID_ARG", 2: "HMVM_ERR_BAD_BLOCK", 3: "HMVM_ERR_OVERLAP",
     4: "HMVM_ERR_NONFINITE", 5: "HMVM_ERR_ALLOC", 6: "HMVM_ERR_DEVICE",
     7: "HMVM_ERR_UNSUPPORTED", 8: "HMVM_ERR_LAYOUT", 9: "HMVM_ERR_KERNEL",
+    10: "HMVM_ERR_IO", 11: "HMVM_ERR_CORRUPT",
 }
 OP_NORMAL, OP_TRANSPOSED, OP_ADJOINT, OP_CONJUGATE = 0, 1, 2, 3
 BLOCK_DENSE, BLOCK_LOWRANK, BLOCK_LOWRANK_SV = 0, 1, 2
@@ -117,6 +118,11 @@ HMVM_SYMBOLS = [
     ("hmvm_inv_error_2", _i32, [_vp, _vp, C.c_double, _i64, _vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
     ("hmvm_linear_iteration", _i32, [_vp, _vp, _vp, _vp, _vp, C.c_double, _i64, _vp, C.POINTER(_i64),
                                      C.POINTER(C.c_double)]),
+    ("hmvm_hmatrix_write", _i32, [C.c_char_p, _vp, _i64, _i64, _i64, _vp, _i64]),
+    ("hmvm_hmatrix_read_info", _i32, [C.c_char_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
+                                      C.POINTER(_i64)]),
+    ("hmvm_hmatrix_read", _i32, [C.c_char_p, _vp, _i64, _vp, _i64]),
+    ("hmvm_plan_create_from_file", _i32, [C.c_char_p, _vp, _i32, C.POINTER(_vp)]),
     ("hmvm_error_string", C.c_char_p, [_i32]),
 ]
 

@@ -49,7 +49,7 @@ def build_hmvm() -> Path:
     LIBDIR.mkdir(exist_ok=True)
     dev = CSRC / "device"
     objs = []
-    for src in ("plan_build.cpp", "hmvm.hip", "mrhs.hip", "batch.hip", "vecops.hip"):
+    for src in ("plan_build.cpp", "hmfile.cpp", "hmvm.hip", "mrhs.hip", "batch.hip", "vecops.hip"):
         obj = LIBDIR / (Path(src).stem + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         flags = DEVICE_FLAGS + [f"--offload-arch={ARCH}"] if src.endswith(".hip") else []

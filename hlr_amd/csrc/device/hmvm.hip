@@ -636,6 +636,8 @@ const char* hmvm_error_string(int code) {
     case HMVM_ERR_UNSUPPORTED: return "unsupported combination";
     case HMVM_ERR_LAYOUT: return "bad leading dimension or alignment";
     case HMVM_ERR_KERNEL: return "kernel reported a failed check";
+    case HMVM_ERR_IO: return "file missing, short, or not an H-matrix file";
+    case HMVM_ERR_CORRUPT: return "H-matrix file fails its checksums";
     default: return "unknown error";
     }
 }

@@ -9,6 +9,7 @@
 //   hlr::hip::norm::spectral(arith, A, ...)    (arith/norm.hh:89-101)
 //   hlr::hip::norm::inv_error_2(arith, A, W)   (arith/norm.hh:115-137)
 //   hlr::hip::linear_iteration(A, W, b, x, ...) (programs/iterrefine.hh:20-80)
+//   hlr::hip::io::write / read                 (utils/io.hh:59-80)
 // with y += alpha op(M) x, alpha == 0 a no-op, op in {apply_normal,
 // apply_conjugate, apply_transposed, apply_adjoint}, and errors thrown as
 // std::runtime_error (HLR_ASSERT/HLR_ERROR, utils/log.hh:40-67).  The matrix
@@ -89,6 +90,11 @@ public:
               "hmvm_plan_create");
         check(hmvm_plan_get_info(p_, &info_), "hmvm_plan_get_info");
     }
+    // from an H-matrix file written by io::write
+    explicit hmatrix(const std::string& filename, const hmvm_plan_opts* opts = nullptr, int device = 0) {
+        check(hmvm_plan_create_from_file(filename.c_str(), opts, device, &p_), "hmvm_plan_create_from_file");
+        check(hmvm_plan_get_info(p_, &info_), "hmvm_plan_get_info");
+    }
     hmatrix(const hmatrix&) = delete;
     hmatrix& operator=(const hmatrix&) = delete;
     ~hmatrix() { if (p_) hmvm_plan_destroy(p_); }
@@ -122,6 +128,28 @@ void mul_vec(const value_t alpha, const matop_t op, const hmatrix& M, const devi
                      int64_t(y.length()), x.ncols(), vec_dtype<value_t>(), stream),
           "hmvm_apply");
 }
+
+// H-matrix files: the flattened leaves and FP64 arena that hmatrix takes
+// (in place of io::hpro::write / read, utils/io.hh:59-80)
+namespace io {
+
+inline void write(const std::vector<hmvm_block>& blocks, const std::vector<double>& arena, int64_t nrows,
+                  int64_t ncols, const std::string& filename) {
+    check(hmvm_hmatrix_write(filename.c_str(), blocks.data(), int64_t(blocks.size()), nrows, ncols, arena.data(),
+                             int64_t(arena.size())),
+          "hmvm_hmatrix_write");
+}
+
+inline void read(const std::string& filename, std::vector<hmvm_block>& blocks, std::vector<double>& arena,
+                 int64_t& nrows, int64_t& ncols) {
+    int64_t nb = 0, na = 0;
+    check(hmvm_hmatrix_read_info(filename.c_str(), &nrows, &ncols, &nb, &na), "hmvm_hmatrix_read_info");
+    blocks.resize(size_t(nb));
+    arena.resize(size_t(na));
+    check(hmvm_hmatrix_read(filename.c_str(), blocks.data(), nb, arena.data(), na), "hmvm_hmatrix_read");
+}
+
+} // namespace io
 
 namespace tlr {
 

@@ -10,6 +10,10 @@
 // usage: mvm-hip [--app log|logkernel|materncov|invr] [-n N] [--ntile K]
 //                [--eta E] [--eps E] [--nbench B] [--prec fp64|fp32|valr|bfp]
 //                [--tlr] [--xrand] [--op normal|transposed]
+//                [--save FILE | --load FILE]
+// --save stores the generated matrix as an H-matrix file; --load reads one
+// instead of generating (the problem options then only label the output,
+// except --eps, which sets the plan's tolerance).
 
 #include <algorithm>
 #include <chrono>
@@ -28,7 +32,7 @@ using namespace hlr::hip;
 int main(int argc, char** argv) {
     hlrgen_config cfg;
     hlrgen_default_config(&cfg);
-    std::string app = "log", prec = "fp64", opname = "normal";
+    std::string app = "log", prec = "fp64", opname = "normal", save, load;
     int nbench = 5;
     bool xrand = false;
     for (int i = 1; i < argc; ++i) {
@@ -47,6 +51,8 @@ int main(int argc, char** argv) {
         else if (a == "--op") opname = next();
         else if (a == "--tlr") cfg.format = HLRGEN_FORMAT_TLR;
         else if (a == "--xrand") xrand = true;
+        else if (a == "--save") save = next();
+        else if (a == "--load") load = next();
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     if (app == "log") { cfg.kernel = HLRGEN_KERNEL_LOG; cfg.geometry = HLRGEN_GEOM_LINE; }
@@ -57,15 +63,32 @@ int main(int argc, char** argv) {
     const matop_t op = (opname == "transposed") ? apply_transposed : apply_normal;
 
     try {
-        hlrgen_matrix* gm = nullptr;
-        check(hlrgen_build(&cfg, &gm), "hlrgen_build");
-        const int64_t n = hlrgen_n(gm), nb = hlrgen_nblocks(gm);
-        std::vector<hmvm_block> blocks(hlrgen_blocks(gm), hlrgen_blocks(gm) + nb);
-        std::vector<double> arena(hlrgen_arena(gm), hlrgen_arena(gm) + hlrgen_arena_len(gm));
-        std::printf("problem   : %s, n = %lld, ntile = %d, eta = %g, eps = %g, %s\n", app.c_str(), (long long)n,
-                    cfg.ntile, cfg.eta, cfg.eps, cfg.format == HLRGEN_FORMAT_TLR ? "TLR" : "H");
-        std::printf("build     : %lld leaves in %.2f s\n", (long long)nb, hlrgen_build_seconds(gm));
-        hlrgen_free(gm);
+        int64_t n = 0, ncols = 0;
+        std::vector<hmvm_block> blocks;
+        std::vector<double> arena;
+        if (!load.empty()) {
+            const auto t0 = std::chrono::steady_clock::now();
+            io::read(load, blocks, arena, n, ncols);
+            if (ncols != n) throw std::runtime_error("mvm-hip: the stored matrix is not square");
+            std::printf("problem   : %s, n = %lld\n", load.c_str(), (long long)n);
+            std::printf("read      : %lld leaves in %.2f s\n", (long long)blocks.size(),
+                        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        } else {
+            hlrgen_matrix* gm = nullptr;
+            check(hlrgen_build(&cfg, &gm), "hlrgen_build");
+            n = ncols = hlrgen_n(gm);
+            const int64_t nb = hlrgen_nblocks(gm);
+            blocks.assign(hlrgen_blocks(gm), hlrgen_blocks(gm) + nb);
+            arena.assign(hlrgen_arena(gm), hlrgen_arena(gm) + hlrgen_arena_len(gm));
+            std::printf("problem   : %s, n = %lld, ntile = %d, eta = %g, eps = %g, %s\n", app.c_str(),
+                        (long long)n, cfg.ntile, cfg.eta, cfg.eps, cfg.format == HLRGEN_FORMAT_TLR ? "TLR" : "H");
+            std::printf("build     : %lld leaves in %.2f s\n", (long long)nb, hlrgen_build_seconds(gm));
+            hlrgen_free(gm);
+        }
+        if (!save.empty()) {
+            io::write(blocks, arena, n, n, save);
+            std::printf("saved     : %s\n", save.c_str());
+        }
 
         hmvm_plan_opts opts{};
         opts.precision = prec == "fp32" ? HMVM_PREC_FP32 : prec == "valr" ? HMVM_PREC_VALR

@@ -122,6 +122,10 @@ class HMatrix:
                                        out.ctypes.data), "hlrgen_eval")
         return out
 
+    def save(self, path) -> None:
+        """Store the leaves and the FP64 arena in an H-matrix file."""
+        save_hmatrix(path, self.blocks, self.arena, self.nrows, self.ncols)
+
     def stats(self) -> dict:
         b = self.blocks
         dense = b["kind"] == N.BLOCK_DENSE
@@ -144,3 +148,40 @@ def random_vector(n: int, seed: int, lo: float = -1.0, hi: float = 1.0, gaussian
     out = np.empty(n)
     N.hlrgen().hlrgen_random_vector(seed, n, lo, hi, 1 if gaussian else 0, out.ctypes.data)
     return out
+
+
+class StoredHMatrix:
+    """An H-matrix read back from an H-matrix file: leaves + FP64 arena
+    (numpy-owned), with the attributes of HMatrix that plans and the oracle
+    use (blocks, arena, n, nrows, ncols)."""
+
+    def __init__(self, blocks: np.ndarray, arena: np.ndarray, nrows: int, ncols: int):
+        self.blocks, self.arena = blocks, arena
+        self.nrows, self.ncols = nrows, ncols
+        self.n = nrows
+
+    def save(self, path) -> None:
+        save_hmatrix(path, self.blocks, self.arena, self.nrows, self.ncols)
+
+
+def save_hmatrix(path, blocks: np.ndarray, arena: np.ndarray, nrows: int, ncols: int) -> None:
+    """Write leaves + FP64 arena as an H-matrix file (hmvm_hmatrix_write)."""
+    blocks = np.ascontiguousarray(blocks, dtype=N.block_dtype())
+    arena = np.ascontiguousarray(arena, dtype=np.float64)
+    N.check(N.hmvm().hmvm_hmatrix_write(str(path).encode(), blocks.ctypes.data if len(blocks) else None,
+                                        len(blocks), int(nrows), int(ncols),
+                                        arena.ctypes.data if arena.size else None, arena.size),
+            "hmvm_hmatrix_write")
+
+
+def load_hmatrix(path) -> StoredHMatrix:
+    """Read an H-matrix file (hmvm_hmatrix_read_info + hmvm_hmatrix_read)."""
+    nr, nc, nb, na = (C.c_int64() for _ in range(4))
+    p = str(path).encode()
+    N.check(N.hmvm().hmvm_hmatrix_read_info(p, C.byref(nr), C.byref(nc), C.byref(nb), C.byref(na)),
+            "hmvm_hmatrix_read_info")
+    blocks = np.zeros(nb.value, dtype=N.block_dtype())
+    arena = np.zeros(na.value)
+    N.check(N.hmvm().hmvm_hmatrix_read(p, blocks.ctypes.data if nb.value else None, nb.value,
+                                       arena.ctypes.data if na.value else None, na.value), "hmvm_hmatrix_read")
+    return StoredHMatrix(blocks, arena, int(nr.value), int(nc.value))

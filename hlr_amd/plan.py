@@ -72,6 +72,15 @@ class Plan:
     def from_hmatrix(cls, hm, **kw) -> "Plan":
         return cls(hm.blocks, hm.arena, hm.nrows, hm.ncols, **kw)
 
+    @classmethod
+    def from_file(cls, path, **kw) -> "Plan":
+        """Plan of the matrix in an H-matrix file, in any storage precision
+        (the file is read into host memory, then the plan is built from it;
+        same options as the constructor)."""
+        from .hmatrix import load_hmatrix
+        hm = load_hmatrix(path)
+        return cls(hm.blocks, hm.arena, hm.nrows, hm.ncols, **kw)
+
     def close(self) -> None:
         if self._p:
             N.hmvm().hmvm_plan_destroy(self._p)

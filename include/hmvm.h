@@ -41,7 +41,9 @@ enum {
     HMVM_ERR_DEVICE         = 6,   /* HIP runtime error                         */
     HMVM_ERR_UNSUPPORTED    = 7,   /* combination not implemented               */
     HMVM_ERR_LAYOUT         = 8,   /* leading dimension too small / misaligned  */
-    HMVM_ERR_KERNEL         = 9    /* a kernel reported a failed internal check */
+    HMVM_ERR_KERNEL         = 9,   /* a kernel reported a failed internal check */
+    HMVM_ERR_IO             = 10,  /* file missing, short, or not an H-matrix file */
+    HMVM_ERR_CORRUPT        = 11   /* H-matrix file fails its checksums         */
 };
 
 /* matrix operation, values as Hpro::matop_t */
@@ -344,6 +346,42 @@ int hmvm_linear_iteration(const hmvm_plan* A, const hmvm_plan* W,
                           const double* b, double* x, const double* sol,
                           double tol, int64_t max_steps, void* stream,
                           int64_t* steps, double* history);
+
+/* ------------------------------------------------------------------------ */
+/* on-disk H-matrix                                                         */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Store a flattened H- or TLR-matrix (the leaf records and the FP64 arena
+ * that hmvm_plan_create takes) in a file, replacing hlr::io::hpro::write /
+ * read (utils/io.hh:59-80), which go through HLIBpro's own file formats.
+ * The file is a 72-byte header (magic "HMVMHMAT", version, nrows, ncols,
+ * nblocks, arena_len and checksums of the header, the records and the
+ * arena), nblocks records of 9 int64 (row_ofs, col_ofs, nrows, ncols, rank,
+ * kind, off_a, off_b, off_s), then arena_len doubles, little-endian.
+ * Writing does not validate the leaves (plan creation does); a failed write
+ * removes the file.
+ */
+int hmvm_hmatrix_write(const char* path, const hmvm_block* blocks,
+                       int64_t nblocks, int64_t nrows, int64_t ncols,
+                       const double* arena, int64_t arena_len);
+
+/* sizes stored in an H-matrix file (any pointer may be NULL) */
+int hmvm_hmatrix_read_info(const char* path, int64_t* nrows, int64_t* ncols,
+                           int64_t* nblocks, int64_t* arena_len);
+
+/*
+ * Read an H-matrix file into caller buffers of exactly the stored sizes
+ * (HMVM_ERR_INVALID_ARG otherwise).  HMVM_ERR_IO: missing or short file,
+ * or not this format; HMVM_ERR_CORRUPT: a checksum fails, a record field
+ * is out of range, or data follows the arena.
+ */
+int hmvm_hmatrix_read(const char* path, hmvm_block* blocks, int64_t nblocks,
+                      double* arena, int64_t arena_len);
+
+/* hmvm_hmatrix_read + hmvm_plan_create, with the host copies freed after */
+int hmvm_plan_create_from_file(const char* path, const hmvm_plan_opts* opts,
+                               int device, hmvm_plan** out);
 
 const char* hmvm_error_string(int code);
 
