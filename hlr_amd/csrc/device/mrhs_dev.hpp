@@ -183,9 +183,16 @@ __device__ inline void psum1_item(const DevPlan& D, const double* __restrict__ P
     const int r = int(g % R);
     const int32_t e = D.ps_e[s], r0 = D.ps_r0[s];
     const int32_t o = D.pt_off[e], k = D.tstride[e];
-    const int32_t r1 = min(r0 + kSegLen, D.pt_cnt[e]);
+    const int32_t cnt = min(kSegLen, D.pt_cnt[e] - r0);
+    // all of the segment's loads in flight at once, then the sum in order
+    const double* src = Pb + (int64_t(o) + int64_t(r0) * k) * R + r;
+    double v[kSegLen];
+#pragma unroll
+    for (int u = 0; u < kSegLen; ++u) v[u] = (u < cnt) ? src[int64_t(u) * k * R] : 0.0;
     double a = 0;
-    for (int32_t j = r0; j < r1; ++j) a += Pb[(int64_t(o) + int64_t(j) * k) * R + r];
+#pragma unroll
+    for (int u = 0; u < kSegLen; ++u)
+        if (u < cnt) a += v[u];
     tmp[s * R + r] = a;
 }
 
@@ -194,8 +201,16 @@ __device__ inline void psum2_item(const DevPlan& D, const double* __restrict__ t
     const int64_t e = g / R;
     const int r = int(g % R);
     const int32_t p = D.ps_pos0[e], n = D.ps_nseg[e], k = D.tstride[e];
+    const double* src = tmp + int64_t(p) * R + r;
     double a = 0;
-    for (int32_t j = 0; j < n; ++j) a += tmp[(int64_t(p) + int64_t(j) * k) * R + r];
+    for (int32_t j0 = 0; j0 < n; j0 += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = (j0 + u < n) ? src[int64_t(j0 + u) * k * R] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + u < n) a += v[u];
+    }
     t[e * R + r] = alpha * D.tmul[e] * a;
 }
 
@@ -230,19 +245,55 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     const int row = 16 * wave + (lane & 15);
     const int kk = lane >> 4, n = lane & 15;
     const bytep base = elem_addr<T>(P.arena[M::F], Ch.data_off) + int64_t(row < m ? row : 0) * 16;
+    // The z values of a staged block (rhs fastest: contiguous reads of XT
+    // and t).  When RS divides kThr, a thread keeps one rhs index r and
+    // stages columns c0 + j kThr / RS in rounds of ZB: a round issues its
+    // column-source loads, then its value loads, so that a block waits for
+    // a few load round trips rather than two per value.  Otherwise (RS = 48)
+    // a plain loop, which needs fewer registers there.
+    constexpr bool kFixedR = (kThr % RS) == 0;
+    constexpr int NI = (SC * RS + kThr - 1) / kThr;
+    constexpr int ZB = NI < 4 ? NI : 4;
+    const int c0 = tid / RS, r0 = tid % RS;
     for (int gb0 = 0; gb0 < Ch.ngroups; gb0 += GB) {
-        __syncthreads();
-        for (int idx = tid; idx < SC * RS; idx += kThr) {
-            const int c = idx / RS, r = idx - c * RS;    // rhs fastest: contiguous reads of XT and t
-            if (r >= Rp) continue;
-            const int col = gb0 * W + c;
-            double z = 0;
-            if (col < ncol && r < R) {
-                const int32_t s = P.colsrc[Ch.col0 + col];
-                z = (s >= 0) ? alpha * XTb[int64_t(s) * R + r] : (s == kPadCol ? 0.0 : t[int64_t(~s) * R + r]);
-                if (E::kScaled) z *= double(P.zscale[Ch.col0 + col]);
+        const int64_t cb = int64_t(Ch.col0) + gb0 * W;
+        const int lim = min(SC, ncol - gb0 * W);         // columns of this block that exist
+        __syncthreads();                                 // the MFMAs of the block before are done with zs
+        if constexpr (kFixedR) {
+#pragma unroll
+            for (int j0 = 0; j0 < NI; j0 += ZB) {
+                int32_t src[ZB];
+                float sc[ZB];
+#pragma unroll
+                for (int u = 0; u < ZB; ++u) {
+                    const int c = c0 + (j0 + u) * (kThr / RS);
+                    src[u] = (j0 + u < NI && c < lim) ? P.colsrc[cb + c] : kPadCol;
+                    if constexpr (E::kScaled) sc[u] = (j0 + u < NI && c < lim) ? P.zscale[cb + c] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < ZB; ++u) {
+                    const int c = c0 + (j0 + u) * (kThr / RS);
+                    const int32_t s = src[u];
+                    double z = 0;
+                    if (s != kPadCol && r0 < R) {
+                        z = (s >= 0) ? alpha * XTb[int64_t(s) * R + r0] : t[int64_t(~s) * R + r0];
+                        if constexpr (E::kScaled) z *= double(sc[u]);
+                    }
+                    if (j0 + u < NI && c < SC) zs[c * ZS + r0] = A(z);
+                }
             }
-            zs[c * ZS + r] = A(z);
+        } else {
+            for (int idx = tid; idx < SC * RS; idx += kThr) {
+                const int c = idx / RS, r = idx - c * RS;
+                if (r >= Rp) continue;
+                double z = 0;
+                if (c < lim && r < R) {
+                    const int32_t s = P.colsrc[cb + c];
+                    z = (s >= 0) ? alpha * XTb[int64_t(s) * R + r] : (s == kPadCol ? 0.0 : t[int64_t(~s) * R + r]);
+                    if constexpr (E::kScaled) z *= double(P.zscale[cb + c]);
+                }
+                zs[c * ZS + r] = A(z);
+            }
         }
         __syncthreads();
         const int gend = min(gb0 + GB, int(Ch.ngroups));
