@@ -48,16 +48,10 @@ __global__ __launch_bounds__(kThr) void k_vfwd_mm(DevPlan P, const double* __res
     vseg_mm_dispatch<NTM, FAM>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
 }
 
-__global__ __launch_bounds__(256) void k_psum1_mm(DevPlan D, const double* __restrict__ Pb, double* __restrict__ tmp,
-                                                  int R) {
+__global__ __launch_bounds__(256) void k_psum_mm(DevPlan D, const double* __restrict__ Pb, double* __restrict__ t,
+                                                 int R, double alpha) {
     const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g < D.nps * R) psum1_item(D, Pb, tmp, R, g);
-}
-
-__global__ __launch_bounds__(256) void k_psum2_mm(DevPlan D, const double* __restrict__ tmp, double* __restrict__ t,
-                                                  int R, double alpha) {
-    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g < D.ntent * R) psum2_item(D, tmp, t, R, alpha, g);
+    if (g < D.ntent * R) psum_mm_item(D, Pb, t, R, alpha, g);
 }
 
 template <int NTM, typename YT, int FAM>
@@ -77,10 +71,9 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
     if (R < 2 || R > kMaxRp) return 1;
     if (fam != kFamHw && fam != kFamBfp) return 1;
     const int64_t pad = 32;
-    // workspace: partials | stage-1 sums | t | XT (ncols x R)
+    // workspace: partials | t | XT (ncols x R)
     double* part = ws;
-    double* tmp = part + ((D.psize * R + pad - 1) / pad) * pad;
-    double* t = tmp + ((D.nps * R + pad - 1) / pad) * pad;
+    double* t = part + ((D.psize * R + pad - 1) / pad) * pad;
     double* xt = t + ((D.ntent * R + pad - 1) / pad) * pad + pad;
     auto blocks = [](int64_t n, int per) { return unsigned((n + per - 1) / per); };
     const int Rp = (R + 15) & ~15;
@@ -123,18 +116,15 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
         }
     };
     if (D.nvitems > 0) by_tiles(vpass);
-    if (D.ntent > 0) {
-        if (D.nps > 0) k_psum1_mm<<<blocks(D.nps * R, 256), 256, 0, st>>>(D, part, tmp, R);
-        k_psum2_mm<<<blocks(D.ntent * R, 256), 256, 0, st>>>(D, tmp, t, R, alpha);
-    }
+    if (D.ntent > 0) k_psum_mm<<<blocks(D.ntent * R, 256), 256, 0, st>>>(D, part, t, R, alpha);
     if (D.nrp > 0) by_tiles(rpass);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
 int64_t mrhs_ws_doubles(const DevPlan& D, int R) {
     const int64_t pad = 32;
-    return ((D.psize * R + pad - 1) / pad) * pad + ((D.nps * R + pad - 1) / pad) * pad +
-           ((D.ntent * R + pad - 1) / pad) * pad + pad + ((D.ncols * R + pad - 1) / pad) * pad;
+    return ((D.psize * R + pad - 1) / pad) * pad + ((D.ntent * R + pad - 1) / pad) * pad + pad +
+           ((D.ncols * R + pad - 1) / pad) * pad;
 }
 
 } // namespace hmvm

@@ -11,7 +11,7 @@ constexpr int kMrhsMax = 64;   // right-hand sides per launch sequence
 
 // Y += alpha A X for 2 <= R <= kMrhsMax right-hand sides; vdt 0: FP64
 // vectors, 1: FP32; fam: the plan's format family (kFamHw or kFamBfp).
-// ws holds mrhs_ws_doubles(D, R) doubles (partials, partial sums, t and the
+// ws holds mrhs_ws_doubles(D, R) doubles (partials, t and the
 // transposed copy of X).  Returns 0 on success.
 int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, int64_t ldx, void* y, int64_t ldy,
                       int R, int vdt, double* ws, hipStream_t st);

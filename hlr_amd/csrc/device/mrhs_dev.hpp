@@ -214,6 +214,32 @@ __device__ inline void psum2_item(const DevPlan& D, const double* __restrict__ t
     t[e * R + r] = alpha * D.tmul[e] * a;
 }
 
+// t(e, r) = alpha S_e sum of the entry's partials in one pass, in the order
+// of psum1 + psum2 (each segment of kSegLen partials summed in order from
+// zero, then the segment sums in order), so the result is bitwise theirs;
+// a segment's loads are issued together
+__device__ inline void psum_mm_item(const DevPlan& D, const double* __restrict__ Pb, double* __restrict__ t, int R,
+                                    double alpha, int64_t g) {
+    const int64_t e = g / R;
+    const int r = int(g % R);
+    const int32_t o = D.pt_off[e], k = D.tstride[e], cnt = D.pt_cnt[e];
+    const double* src = Pb + int64_t(o) * R + r;
+    const int64_t step = int64_t(k) * R;
+    double a = 0;
+    for (int32_t r0 = 0; r0 < cnt; r0 += kSegLen) {
+        const int32_t n = min(kSegLen, cnt - r0);
+        double v[kSegLen];
+#pragma unroll
+        for (int u = 0; u < kSegLen; ++u) v[u] = (u < n) ? src[(int64_t(r0) + u) * step] : 0.0;
+        double s = 0;
+#pragma unroll
+        for (int u = 0; u < kSegLen; ++u)
+            if (u < n) s += v[u];
+        a += s;
+    }
+    t[e * R + r] = alpha * D.tmul[e] * a;
+}
+
 // ---------------------------------------------------------------------------
 // R pass: one workgroup per row panel, wave w owns rows 16w .. 16w + 15
 // ---------------------------------------------------------------------------
