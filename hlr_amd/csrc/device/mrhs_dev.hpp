@@ -240,6 +240,38 @@ __device__ inline void psum_mm_item(const DevPlan& D, const double* __restrict__
     t[e * R + r] = alpha * D.tmul[e] * a;
 }
 
+// the same for two adjacent right-hand sides r, r + 1 (R even, r even): one
+// 16-byte load per partial (P and t are 16-byte aligned, (o R + r) is
+// even), each rhs summed in the order above; loads in batches of 8
+__device__ inline void psum_mm_item2(const DevPlan& D, const double* __restrict__ Pb, double* __restrict__ t, int R,
+                                     double alpha, int64_t g) {
+    const int h = R >> 1;
+    const int64_t e = g / h;
+    const int r = 2 * int(g % h);
+    const int32_t o = D.pt_off[e], k = D.tstride[e], cnt = D.pt_cnt[e];
+    const double2* src = reinterpret_cast<const double2*>(Pb + int64_t(o) * R + r);
+    const int64_t step = int64_t(k) * h;                 // in pairs
+    double ax = 0, ay = 0;
+    for (int32_t r0 = 0; r0 < cnt; r0 += kSegLen) {
+        const int32_t n = min(kSegLen, cnt - r0);
+        double sx = 0, sy = 0;
+#pragma unroll
+        for (int b = 0; b < kSegLen; b += 8) {
+            if (b >= n) break;
+            double2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = (b + u < n) ? src[(int64_t(r0) + b + u) * step] : double2{0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (b + u < n) { sx += v[u].x; sy += v[u].y; }
+        }
+        ax += sx;
+        ay += sy;
+    }
+    const double f = alpha * D.tmul[e];
+    reinterpret_cast<double2*>(t + e * R + r)[0] = double2{f * ax, f * ay};
+}
+
 // ---------------------------------------------------------------------------
 // R pass: one workgroup per row panel, wave w owns rows 16w .. 16w + 15
 // ---------------------------------------------------------------------------
