@@ -145,6 +145,29 @@ def test_multi_rhs_matrix_cores(cuda, sphere16k, prec, nrhs):
         assert rel(got[r] - Y0[r], Yr[r] - Y0[r]) <= tol, r
 
 
+@pytest.mark.parametrize("nrhs", [16, 17])
+def test_multi_rhs_workspace_alignment(cuda, sphere16k, nrhs):
+    """A caller workspace that is only 8-byte aligned takes the partial sums
+    one right-hand side per thread (pairs need 16-byte loads); both give
+    bitwise the product computed with the library's own workspace, as the
+    summation order is the same."""
+    import torch
+    hm = sphere16k
+    plan = Plan.from_hmatrix(hm, precision="valr", eps=1e-6)
+    X = torch.as_tensor(np.stack([random_vector(hm.n, 300 + r) for r in range(nrhs)]), device=cuda)
+    ref = torch.zeros_like(X)
+    print(f"[gpu] mrhs workspace alignment nrhs={nrhs}", flush=True)
+    plan.apply(1.0, N.OP_NORMAL, X, ref)
+    nd = plan.workspace_bytes(N.OP_NORMAL, nrhs) // 8 + 1
+    buf = torch.empty(nd + 1, dtype=torch.float64, device=cuda)
+    for off in (0, 1):                      # 16-byte aligned, then 8-byte aligned
+        y = torch.zeros_like(X)
+        plan.apply_ws(1.0, N.OP_NORMAL, X, y, buf[off:off + nd])
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref), off
+        assert float(y.abs().max()) > 0
+
+
 def test_fp32_vectors(cuda, log1d):
     import torch
     hm = log1d
