@@ -2,7 +2,7 @@
 
 `Batch(plans, nrhs)` collects forward products y_i += alpha_i A_i x_i of many
 plans with per-member numbers of right-hand sides; `apply` runs the work of
-all members as at most thirteen stream-ordered kernels (one unit of work per
+all members as at most twelve stream-ordered kernels (one unit of work per
 workgroup),
 `capture`/`replay` record that sequence in a HIP graph, `tune` runs the
 in-project autotuner over unit order and partial-sum granularity.

@@ -2,7 +2,7 @@
 // one fixed sequence of launches.
 //
 // Every member keeps its own plan.  A batch lists the work of all members as
-// units of seven kinds; one kernel per kind (and per tile count for VM and
+// units of six kinds; one kernel per kind (and per tile count for VM and
 // RM) runs every unit of that kind of every member, one unit per workgroup:
 //   XT  transposed copy of X (64 indices) for members with several
 //       right-hand sides
@@ -10,14 +10,14 @@
 //       per wave)
 //   VM  column-panel pass on the matrix cores (one segment, nrhs >= 2; one
 //       kernel per count of 16-wide right-hand-side tiles)
-//   P1  first stage of the partial sums (a range of (segment, rhs) items)
-//   P2  second stage (a range of (entry, rhs) items)
+//   PS  partial sums (a range of (entry, pair of right-hand sides) items,
+//       or (entry, rhs) items for odd R)
 //   R1  row-panel pass, one right-hand side (one row panel)
 //   RM  row-panel pass on the matrix cores (one row panel, nrhs >= 2; one
 //       kernel per tile count)
 // The launches are ordered on the caller's stream, which is the only
 // synchronisation: no workgroup waits for another.  The launch count is at
-// most thirteen whatever the number of members, and the whole sequence
+// most twelve whatever the number of members, and the whole sequence
 // (descriptor upload plus the kernels) can be captured in a HIP graph and
 // replayed.
 
@@ -43,10 +43,10 @@ namespace {
 // matrix-core kinds come in four variants by tiles of 16 right-hand sides
 // (accumulators and LDS staging sized for ceil(R / 16) tiles of the member)
 enum UnitKind : int {
-    K_XT = 0, K_V1, K_VM1, K_VM2, K_VM3, K_VM4, K_P1, K_P2, K_R1, K_RM1, K_RM2, K_RM3, K_RM4, kKinds
+    K_XT = 0, K_V1, K_VM1, K_VM2, K_VM3, K_VM4, K_PS, K_R1, K_RM1, K_RM2, K_RM3, K_RM4, kKinds
 };
 constexpr int kConfigs = 4;                     // {member order, largest first} x {256, 1024} sum items
-constexpr int kSumItems[2] = {256, 1024};       // (segment | entry, rhs) items per P1 / P2 unit
+constexpr int kSumItems[2] = {256, 1024};       // (entry, rhs pair | rhs) items per sum unit
 
 struct BatchMat {
     DevPlan D;
@@ -55,7 +55,6 @@ struct BatchMat {
     int64_t ldx, ldy;
     double alpha;
     double* part;
-    double* tmp;
     double* t;
     double* xt;               // X^T (ncols x R) for R >= 2
     int32_t R;
@@ -99,20 +98,19 @@ __global__ __launch_bounds__(kThreads) void k_bvm(const BatchMat* __restrict__ m
     vseg_mm_dispatch<NTM, FAM>(M.D, M.D.vitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
 }
 
-__global__ __launch_bounds__(kThreads) void k_bp1(const BatchMat* __restrict__ mats,
+// partial sums of one member: items (entry, pair of right-hand sides) when R
+// is even, else (entry, right-hand side); the workspace is 256-byte aligned
+__global__ __launch_bounds__(kThreads) void k_bps(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
     const int64_t g1 = int64_t(w.first) + w.count;
-    for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads) psum1_item(M.D, M.part, M.tmp, M.R, g);
-}
-
-__global__ __launch_bounds__(kThreads) void k_bp2(const BatchMat* __restrict__ mats,
-                                                  const BatchUnit* __restrict__ units) {
-    const BatchUnit w = units[blockIdx.x];
-    const BatchMat& M = mats[w.mat];
-    const int64_t g1 = int64_t(w.first) + w.count;
-    for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads) psum2_item(M.D, M.tmp, M.t, M.R, M.alpha, g);
+    if (M.R % 2 == 0) {
+        for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads)
+            psum_mm_item2(M.D, M.part, M.t, M.R, M.alpha, g);
+    } else {
+        for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads) psum_mm_item(M.D, M.part, M.t, M.R, M.alpha, g);
+    }
 }
 
 template <typename XT, typename YT, int FAM>
@@ -216,8 +214,7 @@ void launch_kinds(hmvm_batch* b, hipStream_t st) {
     if (grid(K_VM2)) k_bvm<2, FAM><<<grid(K_VM2), kThreads, b->lds[K_VM2], st>>>(b->d_mats, u + o[K_VM2]);
     if (grid(K_VM3)) k_bvm<3, FAM><<<grid(K_VM3), kThreads, b->lds[K_VM3], st>>>(b->d_mats, u + o[K_VM3]);
     if (grid(K_VM4)) k_bvm<4, FAM><<<grid(K_VM4), kThreads, b->lds[K_VM4], st>>>(b->d_mats, u + o[K_VM4]);
-    if (grid(K_P1)) k_bp1<<<grid(K_P1), kThreads, 0, st>>>(b->d_mats, u + o[K_P1]);
-    if (grid(K_P2)) k_bp2<<<grid(K_P2), kThreads, 0, st>>>(b->d_mats, u + o[K_P2]);
+    if (grid(K_PS)) k_bps<<<grid(K_PS), kThreads, 0, st>>>(b->d_mats, u + o[K_PS]);
     if (grid(K_R1)) k_br1<XT, YT, FAM><<<grid(K_R1), kThreads, b->lds[K_R1], st>>>(b->d_mats, u + o[K_R1]);
     if (grid(K_RM1)) k_brm<1, YT, FAM><<<grid(K_RM1), kThreads, b->lds[K_RM1], st>>>(b->d_mats, u + o[K_RM1]);
     if (grid(K_RM2)) k_brm<2, YT, FAM><<<grid(K_RM2), kThreads, b->lds[K_RM2], st>>>(b->d_mats, u + o[K_RM2]);
@@ -311,7 +308,7 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
         m.D = D;
         m.R = R;
         wsoff[i] = wsz;
-        wsz += pad32(D.psize * R) + pad32(D.nps * R) + pad32(D.ntent * R) + 32 + (R > 1 ? pad32(D.ncols * R) : 0);
+        wsz += pad32(D.psize * R) + pad32(D.ntent * R) + 32 + (R > 1 ? pad32(D.ncols * R) : 0);
         for (int gsel = 0; gsel < 2; ++gsel) {
             auto& K = kinds[gsel];
             if (R == 1) {
@@ -332,12 +329,10 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
                     K[K_XT].push_back({{i, int32_t(i0 / 64), 1, 0}, double(std::min<int64_t>(64, D.ncols - i0)) * R});
             }
             const int items = kSumItems[gsel];
-            for (int64_t k = 0; k < D.nps * R; k += items)
-                K[K_P1].push_back({{i, int32_t(k), int32_t(std::min<int64_t>(items, D.nps * R - k)), 0},
-                                   double(std::min<int64_t>(items, D.nps * R - k))});
-            for (int64_t k = 0; k < D.ntent * R; k += items)
-                K[K_P2].push_back({{i, int32_t(k), int32_t(std::min<int64_t>(items, D.ntent * R - k)), 0},
-                                   double(std::min<int64_t>(items, D.ntent * R - k))});
+            const int64_t nsum = D.ntent * (R % 2 == 0 ? R / 2 : R);
+            for (int64_t k = 0; k < nsum; k += items)
+                K[K_PS].push_back({{i, int32_t(k), int32_t(std::min<int64_t>(items, nsum - k)), 0},
+                                   double(std::min<int64_t>(items, nsum - k))});
             for (int64_t p = 0; p < D.nrp; ++p) {
                 const RowPanel& rp = L.rp[p];
                 double by = 0;
@@ -389,8 +384,7 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
         const int R = b->proto[i].R;
         double* base = b->d_ws + wsoff[i];
         b->proto[i].part = base;
-        b->proto[i].tmp = base + pad32(D.psize * R);
-        b->proto[i].t = b->proto[i].tmp + pad32(D.nps * R);
+        b->proto[i].t = base + pad32(D.psize * R);
         b->proto[i].xt = (R > 1) ? b->proto[i].t + pad32(D.ntent * R) + 32 : nullptr;
     }
     *out = b.release();

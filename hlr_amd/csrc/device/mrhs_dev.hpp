@@ -177,47 +177,10 @@ __device__ inline void vseg_mm_dispatch(const DevPlan& P, const VItem& sg, const
 // partial sums for R right-hand sides (entry-major, rhs innermost)
 // ---------------------------------------------------------------------------
 
-__device__ inline void psum1_item(const DevPlan& D, const double* __restrict__ Pb, double* __restrict__ tmp, int R,
-                                  int64_t g) {
-    const int64_t s = g / R;
-    const int r = int(g % R);
-    const int32_t e = D.ps_e[s], r0 = D.ps_r0[s];
-    const int32_t o = D.pt_off[e], k = D.tstride[e];
-    const int32_t cnt = min(kSegLen, D.pt_cnt[e] - r0);
-    // all of the segment's loads in flight at once, then the sum in order
-    const double* src = Pb + (int64_t(o) + int64_t(r0) * k) * R + r;
-    double v[kSegLen];
-#pragma unroll
-    for (int u = 0; u < kSegLen; ++u) v[u] = (u < cnt) ? src[int64_t(u) * k * R] : 0.0;
-    double a = 0;
-#pragma unroll
-    for (int u = 0; u < kSegLen; ++u)
-        if (u < cnt) a += v[u];
-    tmp[s * R + r] = a;
-}
-
-__device__ inline void psum2_item(const DevPlan& D, const double* __restrict__ tmp, double* __restrict__ t, int R,
-                                  double alpha, int64_t g) {
-    const int64_t e = g / R;
-    const int r = int(g % R);
-    const int32_t p = D.ps_pos0[e], n = D.ps_nseg[e], k = D.tstride[e];
-    const double* src = tmp + int64_t(p) * R + r;
-    double a = 0;
-    for (int32_t j0 = 0; j0 < n; j0 += 4) {
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = (j0 + u < n) ? src[int64_t(j0 + u) * k * R] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (j0 + u < n) a += v[u];
-    }
-    t[e * R + r] = alpha * D.tmul[e] * a;
-}
-
-// t(e, r) = alpha S_e sum of the entry's partials in one pass, in the order
-// of psum1 + psum2 (each segment of kSegLen partials summed in order from
-// zero, then the segment sums in order), so the result is bitwise theirs;
-// a segment's loads are issued together
+// t(e, r) = alpha S_e sum of the entry's partials in one pass: each segment
+// of kSegLen partials summed in order from zero, then the segment sums in
+// order (a fixed order, the same for every schedule and batch); a
+// segment's loads are issued together
 __device__ inline void psum_mm_item(const DevPlan& D, const double* __restrict__ Pb, double* __restrict__ t, int R,
                                     double alpha, int64_t g) {
     const int64_t e = g / R;

@@ -262,7 +262,7 @@ int hmvm_batch_destroy(hmvm_batch* batch);
 /*
  * y_i += alpha_i A_i x_i for every plan i (forward products; nrhs[i] in
  * 1..64 right-hand sides, column-major with ldx[i] / ldy[i]).  The work of
- * all members runs as a fixed sequence of at most thirteen stream-ordered
+ * all members runs as a fixed sequence of at most twelve stream-ordered
  * kernels (transposed copy of X, column-panel pass for one and for several
  * right-hand sides, two partial-sum stages, row-panel pass for one and for
  * several right-hand sides; the matrix-core passes once per count of
