@@ -237,6 +237,7 @@ int launch(hmvm_batch* b, hipStream_t st) {
     switch (b->family) {
     case kFamHw: go(std::integral_constant<int, kFamHw>()); break;
     case kFamBfp: go(std::integral_constant<int, kFamBfp>()); break;
+    case kFamF64: go(std::integral_constant<int, kFamF64>()); break;
     default: go(std::integral_constant<int, kFamAll>()); break;
     }
     return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;
@@ -286,9 +287,13 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
     b->nmat = nplans;
     b->plans.assign(plans, plans + nplans);
     for (int i = 0; i < nplans; ++i) b->nrhs.push_back(nrhs ? nrhs[i] : 1);
-    b->family = nplans > 0 ? plan_layout(plans[0]).family : kFamHw;
-    for (int i = 1; i < nplans; ++i)
-        if (plan_layout(plans[i]).family != b->family) b->family = kFamAll;
+    // the members' common family: FP64-only members (their own
+    // instantiation when all are) fit either family; hardware formats
+    // beside block floating point need the all-format kernels
+    auto fam_of = [](const HostLayout& L) { return L.only_f64() ? int(kFamF64) : L.family; };
+    auto join = [](int a, int c) { return a == c ? a : a == kFamF64 ? c : c == kFamF64 ? a : int(kFamAll); };
+    b->family = nplans > 0 ? fam_of(plan_layout(plans[0])) : kFamHw;
+    for (int i = 1; i < nplans; ++i) b->family = join(b->family, fam_of(plan_layout(plans[i])));
     Guard g(dev);
     if (!g.ok) return HMVM_ERR_DEVICE;
 

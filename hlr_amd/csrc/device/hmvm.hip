@@ -581,7 +581,7 @@ int apply_impl(const hmvm_plan* p, double alpha, int op, const void* x, int64_t 
                          : launch_apply(p, alpha, op, static_cast<const float*>(xr), static_cast<float*>(yr),
                                         static_cast<double*>(ws), st);
             } else {
-                rc = launch_apply_mrhs(p->dp, p->L.family, alpha, xr, ldx, yr, ldy, R, vdt, static_cast<double*>(ws), st) == 0
+                rc = launch_apply_mrhs(p->dp, p->L.only_f64() ? int(kFamF64) : p->L.family, alpha, xr, ldx, yr, ldy, R, vdt, static_cast<double*>(ws), st) == 0
                          ? HMVM_OK : HMVM_ERR_DEVICE;
             }
             if (rc != HMVM_OK) return rc;

@@ -167,12 +167,12 @@ __device__ inline bytep elem_addr(const void* arena, int64_t off) {
 #define HMVM_FMT_SWITCH_FAM(FAM, f, ...)                                                  \
     switch (f) {                                                                          \
     case F64: { using T = double; __VA_ARGS__; } break;                                   \
-    case F32: { using T = float; __VA_ARGS__; } break;                                    \
-    case BF16: if constexpr (FAM != kFamBfp) { using T = uint16_t; __VA_ARGS__; } break;  \
-    case FP8: if constexpr (FAM != kFamBfp) { using T = uint8_t; __VA_ARGS__; } break;    \
-    case B8: if constexpr (FAM != kFamHw) { using T = Bfp<8>; __VA_ARGS__; } break;       \
-    case B16: if constexpr (FAM != kFamHw) { using T = Bfp<16>; __VA_ARGS__; } break;     \
-    default: if constexpr (FAM != kFamHw) { using T = Bfp<25>; __VA_ARGS__; } break;      \
+    case F32: if constexpr (FAM != kFamF64) { using T = float; __VA_ARGS__; } break;      \
+    case BF16: if constexpr (FAM == kFamAll || FAM == kFamHw) { using T = uint16_t; __VA_ARGS__; } break; \
+    case FP8: if constexpr (FAM == kFamAll || FAM == kFamHw) { using T = uint8_t; __VA_ARGS__; } break;   \
+    case B8: if constexpr (FAM == kFamAll || FAM == kFamBfp) { using T = Bfp<8>; __VA_ARGS__; } break;    \
+    case B16: if constexpr (FAM == kFamAll || FAM == kFamBfp) { using T = Bfp<16>; __VA_ARGS__; } break;  \
+    default: if constexpr (FAM == kFamAll || FAM == kFamBfp) { using T = Bfp<25>; __VA_ARGS__; } break;   \
     }
 
 // Runs the statements with T bound to the element type of storage format f.

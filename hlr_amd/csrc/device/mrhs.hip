@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const
 int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, int64_t ldx, void* y, int64_t ldy,
                       int R, int vdt, double* ws, hipStream_t st) {
     if (R < 2 || R > kMaxRp) return 1;
-    if (fam != kFamHw && fam != kFamBfp) return 1;
+    if (fam != kFamHw && fam != kFamBfp && fam != kFamF64) return 1;
     const int64_t pad = 32;
     // workspace: partials | t | XT (ncols x R)
     double* part = ws;
@@ -97,6 +97,7 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
     auto vpass = [&](auto ntm) {
         constexpr int NTM = decltype(ntm)::value;
         if (fam == kFamHw) k_vfwd_mm<NTM, kFamHw><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
+        else if (fam == kFamF64) k_vfwd_mm<NTM, kFamF64><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
         else k_vfwd_mm<NTM, kFamBfp><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
     };
     auto rpass = [&](auto ntm) {
@@ -106,13 +107,13 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
             constexpr int FAM = decltype(famtag)::value;
             k_ufwd_mm<NTM, YT, FAM><<<unsigned(D.nrp), kThr, lds, st>>>(D, alpha, xt, t, static_cast<YT*>(y), ldy, R);
         };
-        if (vdt == 0) {
-            if (fam == kFamHw) go(double(0), std::integral_constant<int, kFamHw>());
-            else go(double(0), std::integral_constant<int, kFamBfp>());
-        } else {
-            if (fam == kFamHw) go(float(0), std::integral_constant<int, kFamHw>());
-            else go(float(0), std::integral_constant<int, kFamBfp>());
-        }
+        auto by_fam = [&](auto ytag) {
+            if (fam == kFamHw) go(ytag, std::integral_constant<int, kFamHw>());
+            else if (fam == kFamF64) go(ytag, std::integral_constant<int, kFamF64>());
+            else go(ytag, std::integral_constant<int, kFamBfp>());
+        };
+        if (vdt == 0) by_fam(double(0));
+        else by_fam(float(0));
     };
     auto by_tiles = [&](auto pass) {
         switch (Rp / 16) {

@@ -244,7 +244,7 @@ __device__ inline void psum_mm_item2(const DevPlan& D, const double* __restrict_
 // a time in LDS (64 x kMmStride); lane i + 16 k loads row 16 w + i of group
 // g0 + k with one 16-byte load, whose element j is A[i][k] of MFMA j
 // (column W (g0 + k) + j), and B[k][n] = Z(W (g0 + k) + j, rhs n).
-template <int NTM, typename T>
+template <int NTM, typename T, bool kUnroll>
 __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alpha, const double* __restrict__ XTb,
                           const double* __restrict__ t, int R, int Rp, void* zraw,
                           typename Mm<T>::V4 (&acc)[NTM], int tid, int wave, int lane) {
@@ -252,9 +252,11 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     using A = typename M::A;
     using E = Elem<T>;
     constexpr int W = M::W;
-    // groups per staged block of at most 64 columns: a multiple of the 4
-    // groups of one MFMA step, so that no step reads past the staged columns
-    constexpr int GB = (64 / W) & ~3;
+    // groups per staged block: a multiple of the 4 groups of one MFMA step,
+    // so that no step reads past the staged columns; the LDS of 64 FP64
+    // columns holds 128 FP32 ones (formats decoded to FP32)
+    constexpr int kMaxCols = 64 * int(sizeof(double) / sizeof(A));
+    constexpr int GB = (kMaxCols / W) & ~3;
     constexpr int SC = GB * W;                           // staged columns per block
     static_assert(GB >= 4, "a staged block holds at least one MFMA step");
     constexpr int NB = MmSteps<W>::value;
@@ -281,8 +283,8 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
         const int lim = min(SC, ncol - gb0 * W);         // columns of this block that exist
         __syncthreads();                                 // the MFMAs of the block before are done with zs
         if constexpr (kFixedR) {
-#pragma unroll
-            for (int j0 = 0; j0 < NI; j0 += ZB) {
+            // one round: ZB column-source loads, then ZB value loads
+            auto round = [&](int j0) {
                 int32_t src[ZB];
                 float sc[ZB];
 #pragma unroll
@@ -302,6 +304,16 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
                     }
                     if (j0 + u < NI && c < SC) zs[c * ZS + r0] = A(z);
                 }
+            };
+            // unrolled up to two rounds in the FP64-only kernel; elsewhere
+            // in a loop, which keeps the registers of one round (the
+            // kernel's registers are those of its largest format path)
+            if constexpr (kUnroll && NI <= 2 * ZB) {
+#pragma unroll
+                for (int j0 = 0; j0 < NI; j0 += ZB) round(j0);
+            } else {
+#pragma unroll 1
+                for (int j0 = 0; j0 < NI; j0 += ZB) round(j0);
             }
         } else {
             for (int idx = tid; idx < SC * RS; idx += kThr) {
@@ -365,7 +377,7 @@ __device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double al
     for (int i = 0; i < NTM; ++i) { acc64[i] = d4{0, 0, 0, 0}; acc32[i] = f4{0, 0, 0, 0}; }
     for (int ci = 0; ci < Rw.nchunks; ++ci) {
         const RChunk Ch = P.rchunks[Rw.chunk0 + ci];
-        HMVM_FMT_SWITCH_FAM(FAM, Ch.fmt, rchunk_mm<NTM, T>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf,
+        HMVM_FMT_SWITCH_FAM(FAM, Ch.fmt, rchunk_mm<NTM, T, FAM == kFamF64>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf,
                                                             RAcc<T>::pick(acc64, acc32), tid, wave, lane))
     }
     // bring the FP64 tiles into the FP32 element order (row 4 kk + e), then

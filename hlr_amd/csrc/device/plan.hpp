@@ -44,7 +44,9 @@ constexpr int kNumBfp = NFMT - kFirstBfp;
 // builder enforces it), so a kernel instantiated for that family carries the
 // decode paths, and the registers, of its own formats alone.  FP64 and FP32
 // belong to both.
-enum FmtFamily : int { kFamAll = 0, kFamHw = 1, kFamBfp = 2 };
+// kFamF64: a plan that stores FP64 values only (the matrix-core kernels
+// have an instantiation of their own for it)
+enum FmtFamily : int { kFamAll = 0, kFamHw = 1, kFamBfp = 2, kFamF64 = 3 };
 
 constexpr int kPanel = 64;          // rows per panel = lanes per wave
 constexpr int kWaves = 4;           // waves per workgroup

@@ -85,6 +85,12 @@ struct HostLayout {
         default: return abfp[f - kFirstBfp].empty() ? nullptr : abfp[f - kFirstBfp].data();
         }
     }
+    // true when every stored value is FP64
+    bool only_f64() const {
+        for (int f = 1; f < NFMT; ++f)
+            if (nvals[f] > 0) return false;
+        return true;
+    }
     void release_arenas() {
         std::vector<double>().swap(a64);
         std::vector<float>().swap(a32);
