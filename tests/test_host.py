@@ -295,3 +295,29 @@ def test_dense_precision_follows_eps(sphere, eps, exact):
     m, n = int(blk["nrows"]), int(blk["ncols"])
     ref = sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T
     assert np.array_equal(D, ref.astype(np.float32).astype(np.float64))
+
+
+def test_consumer_python_wrappers_check_vectors():
+    """linear_iteration takes contiguous float64 vectors of the plan's size
+    (checked before the C call); inv_error_2 on host-only plans reports
+    HMVM_ERR_INVALID_ARG as an HmvmError."""
+    import torch
+    from hlr_amd.plan import inv_error_2, linear_iteration
+    p = Plan(_one_block(), np.ones(16), 4, 4, device=-1)
+    good = torch.zeros(4, dtype=torch.float64)
+    for bad in (torch.zeros(4, dtype=torch.float32), torch.zeros(5, dtype=torch.float64),
+                torch.zeros(8, dtype=torch.float64)[::2], torch.zeros((2, 2), dtype=torch.float64)):
+        with pytest.raises(ValueError):
+            linear_iteration(p, p, bad, good)
+        with pytest.raises(ValueError):
+            linear_iteration(p, p, good, bad)
+    with pytest.raises(ValueError):
+        linear_iteration(p, p, good, good, sol=torch.zeros(3, dtype=torch.float64))
+    with pytest.raises(N.HmvmError) as ei:
+        inv_error_2(p, p, stream=_NoStream())
+    assert ei.value.code == 1
+
+
+class _NoStream:
+    """stands in for a torch stream on a machine without a GPU"""
+    cuda_stream = 0
