@@ -413,14 +413,6 @@ int upload_plan(hmvm_plan* p) {
     UP(tstride, L.tstride);
     UP(vitems, L.vitems);
     UP(vpacks, L.vpacks);
-    UP(ps_e, L.ps_e);
-    UP(ps_r0, L.ps_r0);
-    UP(ps_pos0, L.ps_pos0);
-    UP(ps_nseg, L.ps_nseg);
-    UP(qs_e, L.qs_e);
-    UP(qs_r0, L.qs_r0);
-    UP(qs_pos0, L.qs_pos0);
-    UP(qs_nseg, L.qs_nseg);
     UP(pl, L.pl);
     UP(pl_il, L.pl_il);
     UP(ql, L.ql);
@@ -450,8 +442,6 @@ int upload_plan(hmvm_plan* p) {
     D.ncp = int64_t(L.cp.size());
     D.nvitems = int64_t(L.vitems.size());
     D.nvpacks = int64_t(L.vpacks.size());
-    D.nps = int64_t(L.ps_e.size());
-    D.nqs = int64_t(L.qs_e.size());
     D.nrchunks = int64_t(L.rchunks.size());
     D.ntent = L.ntent;
     D.psize = L.psize;
@@ -480,13 +470,12 @@ inline unsigned blocks_for(int64_t n, int per) { return unsigned((n + per - 1) /
 
 inline int64_t pad32(int64_t v) { return ((v + 31) / 32) * 32; }
 
-// workspace: partials | stage-1 segment sums | t (or s) + 32 spare
+// workspace: partials | t (or s) + 32 spare
 int64_t ws_doubles(const hmvm_plan* p, int op) {
     const bool trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
     const HostLayout& L = p->L;
     const int64_t a = trans ? L.qsize : L.psize;
-    const int64_t sg = trans ? int64_t(L.qs_e.size()) : int64_t(L.ps_e.size());
-    return pad32(a) + pad32(sg) + pad32(L.ntent) + 32;
+    return pad32(a) + pad32(L.ntent) + 32;
 }
 
 // t = alpha S sum(partials), one kernel (k_psum)
@@ -504,10 +493,8 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
     const HostLayout& L = p->L;
     const bool trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
     const int64_t abuf = trans ? L.qsize : L.psize;
-    const int64_t nseg = trans ? D.nqs : D.nps;
     double* part = ws;
-    double* tmp = ws + pad32(abuf);
-    double* t = tmp + pad32(nseg);
+    double* t = ws + pad32(abuf);
     if (ev) (void)hipEventRecord(ev[0], st);
     if (!trans) {
         if (D.nvpacks > 0)

@@ -35,20 +35,12 @@ struct DevPlan {
     const int32_t* qt_cnt;
     const int32_t* tstride;
     const VPack* vpacks;
-    const int32_t* ps_e;
-    const int32_t* ps_r0;
-    const int32_t* ps_pos0;
-    const int32_t* ps_nseg;
-    const int32_t* qs_e;
-    const int32_t* qs_r0;
-    const int32_t* qs_pos0;
-    const int32_t* qs_nseg;
     const SumLane* pl;        // one-kernel sums: lanes per t-entry (see the plan builder)
     const int32_t* pl_il;
     const SumLane* ql;
     const int32_t* ql_il;
     int64_t npl, nql;
-    int64_t nvpacks, nps, nqs;
+    int64_t nvpacks;
     const int32_t* qd_off;
     const int32_t* qd_cnt;
     const void* arena[NFMT];

@@ -441,30 +441,6 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             qpos += k * (bp1[i] - bp0[i]);
         }
         L.psize = pos;
-        // summation segments, block-major (block, segment, column)
-        auto segs = [&](bool fwd, std::vector<int32_t>& se, std::vector<int32_t>& sr0, std::vector<int32_t>& pos0,
-                        std::vector<int32_t>& nsg) {
-            pos0.assign(ne, 0); nsg.assign(ne, 0);
-            int64_t sp = 0;
-            for (int64_t i = 0; i < nb; ++i) {
-                if (L.blk_t0[i] < 0) continue;
-                const int64_t k = blocks[i].rank;
-                const int64_t c = fwd ? (bq1[i] - bq0[i]) : (bp1[i] - bp0[i]);
-                const int64_t ns = (c + kSegLen - 1) / kSegLen;
-                for (int64_t l = 0; l < k; ++l) {
-                    pos0[L.blk_t0[i] + l] = int32_t(sp + l);
-                    nsg[L.blk_t0[i] + l] = int32_t(ns);
-                }
-                for (int64_t j = 0; j < ns; ++j)
-                    for (int64_t l = 0; l < k; ++l) {
-                        se.push_back(int32_t(L.blk_t0[i] + l));
-                        sr0.push_back(int32_t(j * kSegLen));
-                    }
-                sp += ns * k;
-            }
-        };
-        segs(true, L.ps_e, L.ps_r0, L.ps_pos0, L.ps_nseg);
-        segs(false, L.qs_e, L.qs_r0, L.qs_pos0, L.qs_nseg);
         // lanes of the one-kernel sums: an entry's lanes never straddle a wave
         auto lanes = [&](const std::vector<int32_t>& off, const std::vector<int32_t>& cnt, std::vector<SumLane>& le,
                          std::vector<int32_t>& lil) {

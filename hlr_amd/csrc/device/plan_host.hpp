@@ -42,11 +42,6 @@ struct HostLayout {
     std::vector<int32_t> pt_off, pt_cnt;   // forward partials of t-entry e
     std::vector<int32_t> qt_off, qt_cnt;   // transposed partials of t-entry e
     std::vector<int32_t> tstride;          // distance of consecutive partials of e (= rank)
-    // two-stage summation of the partials: stage 1 sums kSegLen partials per
-    // segment (seg_e, seg_r0); stage 2 sums an entry's segment results at
-    // pos0[e] + j * tstride[e], j < nseg[e]
-    std::vector<int32_t> ps_e, ps_r0, ps_pos0, ps_nseg;   // forward (P)
-    std::vector<int32_t> qs_e, qs_r0, qs_pos0, qs_nseg;   // transposed (Q)
     // one-kernel sums for one right-hand side: lanes of t-entry e (at most 64,
     // adjacent, inside one wave); lane i sums segments i, i + L, ... of
     // kSegLen partials and the wave adds up the entry's lanes.  Per lane: e
