@@ -218,7 +218,8 @@ inline int64_t linear_iteration(const hmatrix& A, const hmatrix& W, const device
     if (A.ncols() != A.nrows() || W.nrows() != A.nrows() || W.ncols() != A.nrows() || b.length() != n ||
         x.length() != n || (sol && sol->length() != n))
         throw std::runtime_error("linear_iteration: sizes do not match");
-    if (history) history->assign(size_t(3 * std::max<int64_t>(max_steps, 1)), 0.0);
+    // the library takes max_steps == 0 as 100 steps
+    if (history) history->assign(size_t(3 * (max_steps == 0 ? 100 : std::max<int64_t>(max_steps, 1))), 0.0);
     int64_t steps = 0;
     check(hmvm_linear_iteration(A.plan(), W.plan(), b.data(), x.data(), sol ? sol->data() : nullptr, tol,
                                 max_steps, stream, &steps, history ? history->data() : nullptr),

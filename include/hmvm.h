@@ -337,7 +337,8 @@ int hmvm_inv_error_2(const hmvm_plan* A, const hmvm_plan* W, double atol,
  * for the start value (the reference program starts from r = b at x = 0).
  * Stops when |e| <= tol with sol given, as the reference (tol == 0: 1e-8),
  * or when |r| <= tol |b| with sol == NULL, or after max_steps steps
- * (0: 100).  history (may be NULL, 3 * max_steps doubles) receives per step
+ * (0: 100).  history (may be NULL; 3 doubles per step, 300 when
+ * max_steps == 0) receives per step
  * |r| before the step, |c| and |e| after it (NaN without sol), the three
  * norms the reference prints.  *steps (may be NULL) receives the number of
  * steps taken.  Synchronises `stream` once per step.
