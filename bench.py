@@ -57,6 +57,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=["both", "fp64", "valr", "bfp"], default="both",
                     help="time only one plan (profiling runs); the JSON line needs fp64 and valr")
     ap.add_argument("--no-bfp", action="store_true", help="skip the block floating point plan")
+    ap.add_argument("--mrhs", type=int, default=32,
+                    help="also time each plan with this many right-hand sides per call (0: skip)")
     ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
     ap.add_argument("--workload", choices=["sphere", "batch"], default="sphere",
                     help="sphere: 1/r, n = 262144 (default); batch: 64 Matern matrices, many right-hand sides")
@@ -121,6 +123,35 @@ def main() -> int:
         torch.cuda.synchronize()
         return y.cpu().numpy()
 
+    # several right-hand sides per call (matrix cores), beside the single-RHS
+    # line: ms per call and the product, for each plan before it is closed
+    RM = args.mrhs if (R == 1 and OP == N.OP_NORMAL) else 0
+    xm = torch.as_tensor(np.stack([random_vector(hm.n, 500 + j) for j in range(RM)]), device=dev) if RM > 1 else None
+
+    def time_mrhs(plan, steps=10, warmup=2):
+        if xm is None:
+            return None
+        ws = torch.empty(plan.workspace_bytes(N.OP_NORMAL, RM) // 8 + 1, dtype=torch.float64, device=dev)
+        y = torch.zeros_like(xm)
+        for _ in range(warmup):
+            plan.apply_ws(2.0, N.OP_NORMAL, xm, y, ws)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            plan.apply_ws(2.0, N.OP_NORMAL, xm, y, ws)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        y.zero_()
+        plan.apply(1.0, N.OP_NORMAL, xm, y)
+        torch.cuda.synchronize()
+        out = (ms, y.cpu().numpy())
+        del ws, y
+        torch.cuda.empty_cache()
+        return out
+
     # ---- device copy of known size: achievable HBM bandwidth on this GPU ----
     nbytes = int(args.copy_gib * 2 ** 30)
     src = torch.empty(nbytes // 8, dtype=torch.float64, device=dev).fill_(1.0)
@@ -141,6 +172,7 @@ def main() -> int:
     log(f"device copy: {copy_gbs:.0f} GB/s (read + write of {args.copy_gib:g} GiB)")
 
     ms64 = msmp = None
+    m64 = mmp = mbf = None
     ph64 = phmp = np.zeros(3)
     i64 = imp = None
     y64 = ymp = None
@@ -153,6 +185,7 @@ def main() -> int:
         ms64, ph64 = time_plan(p64, args.steps, args.warmup)
         y64 = result(p64)
         log(f"fp64: {ms64:.3f} ms/MVM  phases {ph64.round(4).tolist()} ms")
+        m64 = time_mrhs(p64)
         p64.close()
         torch.cuda.empty_cache()
 
@@ -166,6 +199,7 @@ def main() -> int:
         msmp, phmp = time_plan(pmp, args.steps, args.warmup)
         ymp = result(pmp)
         log(f"valr: {msmp:.3f} ms/MVM  phases {phmp.round(4).tolist()} ms")
+        mmp = time_mrhs(pmp)
         pmp.close()
 
     # ---- block floating point plan ----
@@ -179,6 +213,7 @@ def main() -> int:
         msbf, phbf = time_plan(pbf, args.steps, args.warmup)
         ybf = result(pbf)
         log(f"bfp: {msbf:.3f} ms/MVM  phases {phbf.round(4).tolist()} ms")
+        mbf = time_mrhs(pbf)
         pbf.close()
         torch.cuda.empty_cache()
         bfp = {"ms_per_step": msbf, "info": ibf, "phases": phbf, "y": ybf}
@@ -283,6 +318,19 @@ def main() -> int:
             rb = ib["bytes_rpass"] + 8.0 * (n + n)
             bf["rpass_gbs"] = rb / (bfp["phases"][2] * 1e-3) / 1e9
         out["bfp"] = bf
+    if RM > 1 and m64 is not None and mmp is not None:
+        # worst relative error over the right-hand sides against the FP64 plan
+        def worst(m):
+            return float(max(np.linalg.norm(m[1][j] - m64[1][j]) / np.linalg.norm(m64[1][j]) for j in range(RM)))
+        mr = {"nrhs": RM, "fp64_ms": m64[0], "valr_ms": mmp[0], "speedup_valr": m64[0] / mmp[0],
+              "rel_err_valr": worst(mmp), "h_mvm_per_s_fp64": 1000.0 * RM / m64[0],
+              "h_mvm_per_s_valr": 1000.0 * RM / mmp[0]}
+        if bfp is not None and mbf is not None:
+            mr.update({"bfp_ms": mbf[0], "speedup_bfp": m64[0] / mbf[0], "rel_err_bfp": worst(mbf),
+                       "h_mvm_per_s_bfp": 1000.0 * RM / mbf[0]})
+        out["mrhs"] = mr
+        log(f"{RM} rhs: fp64 {m64[0]:.3f} ms, valr {mmp[0]:.3f} ms"
+            + (f", bfp {mbf[0]:.3f} ms" if "bfp_ms" in mr else ""))
     print(json.dumps(out), flush=True)
     return 0
 
