@@ -60,7 +60,7 @@ def main() -> int:
     ap.add_argument("--mrhs", type=int, default=32,
                     help="also time each plan with this many right-hand sides per call (0: skip)")
     ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
-    ap.add_argument("--workload", choices=["sphere", "batch"], default="sphere",
+    ap.add_argument("--workload", choices=["sphere", "batch", "grad"], default="sphere",
                     help="sphere: 1/r, n = 262144 (default); batch: 64 Matern matrices, many right-hand sides")
     args = ap.parse_args()
 
@@ -70,6 +70,8 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "grad":
+        return bench_grad(args)
 
     import numpy as np
     import torch
@@ -331,6 +333,92 @@ def main() -> int:
         out["mrhs"] = mr
         log(f"{RM} rhs: fp64 {m64[0]:.3f} ms, valr {mmp[0]:.3f} ms"
             + (f", bfp {mbf[0]:.3f} ms" if "bfp_ms" in mr else ""))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_grad(args) -> int:
+    """Differentiation workload: Matern nu = 1.5, l = 0.1 on n = 131072
+    points in the unit cube, TLR tiles of 512, ACA eps 1e-6, FP32 factors;
+    L = 1/2 |A x - b|^2 with x, b ~ N(0, 1).  One step is the forward
+    product r = A x - b (hmvm_apply_ws) and the adjoint (hmvm_grad):
+    gx = A^T r and the FP64 gradient of every stored factor value, with
+    V^H x recomputed.  Gradient arenas and workspace are allocated before
+    the timed steps."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from hlr_amd import _native as N
+    from hlr_amd.hmatrix import MATERN_TLR, HMatrix, random_vector
+    from hlr_amd.plan import Plan, _stream_handle
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    hm = HMatrix(MATERN_TLR)
+    log(f"matrix: n={hm.n}, {len(hm.blocks)} tiles, built in {time.time() - t0:.1f} s")
+    plan = Plan.from_hmatrix(hm, precision="fp32")
+    info = plan.info()
+    x = torch.as_tensor(random_vector(hm.n, 41, gaussian=True), device=dev)
+    b = torch.as_tensor(random_vector(hm.n, 42, gaussian=True), device=dev)
+    r = torch.empty_like(x)
+    gx = torch.zeros_like(x)
+    elems = [plan.arena_elems(f) for f in range(N.NFMT)]
+    garena = [torch.zeros(e, dtype=torch.float64, device=dev) if e else None for e in elems]
+    ptrs = (C.c_void_p * N.NFMT)(*[C.c_void_p(g.data_ptr()) if g is not None else None for g in garena])
+    ws = torch.empty(plan.workspace_bytes(N.OP_NORMAL, 1) // 8 + 1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = _stream_handle(stream)
+
+    def forward():
+        r.copy_(b).neg_()
+        plan.apply_ws(1.0, N.OP_NORMAL, x, r, ws)          # r = A x - b
+
+    def adjoint():
+        gx.zero_()
+        N.check(N.hmvm().hmvm_grad(plan.handle, C.c_void_p(x.data_ptr()), C.c_void_p(r.data_ptr()),
+                                   C.c_void_p(gx.data_ptr()), ptrs, N.F64, sh), "hmvm_grad")
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    steps, warmup = min(args.steps, 20), min(args.warmup, 3)
+    ms_step = timed(lambda: (forward(), adjoint()), steps, warmup)
+    ms_fwd = timed(forward, steps, warmup)
+    forward()
+    ms_adj = timed(adjoint, steps, warmup)
+    loss = 0.5 * float(torch.dot(r, r))
+    # algorithmic bytes of the adjoint: the V factors (V^H x recomputed), all
+    # factors again (A^T r), 8 B written per stored value, plus the vectors
+    n = hm.n
+    adj_bytes = info["bytes_vpass"] + info["factor_bytes"] + 8.0 * sum(elems) + 8.0 * 4 * n
+    out = {
+        "metric": "loss + gradient steps/s (L = 1/2 |Ax - b|^2, gradients w.r.t. x and every factor)",
+        "value": 1000.0 / ms_step, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 factors, f64 vectors and gradients",
+        "data": "synthetic (seeded points, generated TLR matrix, x and b ~ N(0,1))",
+        "config": {"workload": f"Matern nu=1.5 l=0.1, n={n} in the unit cube, TLR tile 512, ACA eps 1e-6, FP32",
+                   "parallelism": "single GPU"},
+        "forward_ms": ms_fwd, "adjoint_ms": ms_adj, "loss": loss,
+        "factor_bytes": info["factor_bytes"], "gradient_values": int(sum(elems)),
+        "adjoint_algorithmic_bytes": adj_bytes, "adjoint_gbs": adj_bytes / (ms_adj * 1e-3) / 1e9,
+        "forward_gbs": info["datasize"] / (ms_fwd * 1e-3) / 1e9,
+        "cpu_baseline": None,
+    }
+    log(f"step {ms_step:.3f} ms: forward {ms_fwd:.3f} ms, adjoint {ms_adj:.3f} ms "
+        f"({out['adjoint_gbs']:.0f} GB/s algorithmic)")
     print(json.dumps(out), flush=True)
     return 0
 

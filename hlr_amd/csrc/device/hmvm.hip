@@ -295,23 +295,49 @@ __global__ __launch_bounds__(kThreads) void k_grad_r(DevPlan P, const RtItem* __
                                                      const XT* __restrict__ x, const XT* __restrict__ ybar,
                                                      const double* __restrict__ t, double* __restrict__ gR) {
     constexpr int W = Elem<T>::W;
+    constexpr int NZ = kRtSpan * W;                      // columns of one item at most
+    __shared__ double zsh[kWaves][NZ];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t idx = int64_t(blockIdx.x) * kWaves + wave;
-    if (idx >= n) return;
+    if (idx >= n) return;                                // whole wave: no barrier below spans waves
     const RtItem wi = items[idx];
     const RChunk C = P.rchunks[wi.chunk];
     const int gend = min(wi.g0 + kRtSpan, int(C.ngroups));
     const RowPanel R = P.rp[C.panel];
     const int m = R.m;
+    // the item's z values into the wave's LDS slice: all column sources
+    // first, then all values (two load round trips for the item)
+    const int ncol = (gend - wi.g0) * W;
+    const int64_t c0 = C.col0 + int64_t(wi.g0) * W;
+    double* zs = zsh[wave];
+    constexpr int PER = (NZ + 63) / 64;
+    int32_t src[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int j = lane + 64 * u;
+        src[u] = (j < ncol) ? P.colsrc[c0 + j] : kPadCol;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int j = lane + 64 * u;
+        const int32_t sj = src[u];
+        const double z = (sj >= 0) ? double(x[sj]) : (sj == kPadCol ? 0.0 : t[~sj]);
+        if (j < NZ) zs[j] = z;
+    }
+    wave_sync();
     if (lane >= m) return;
     const double yb = double(ybar[R.row0 + lane]);
+    // a lane's W values of a group are adjacent (16-byte stores when W is
+    // even: offsets are multiples of W), consecutive lanes adjacent
     for (int g = wi.g0; g < gend; ++g) {
+        const double* zg = zs + (g - wi.g0) * W;
+        double* d = gR + C.data_off + (int64_t(g) * m + lane) * W;
+        if constexpr (W % 2 == 0) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const int64_t gc = C.col0 + int64_t(g) * W + w;
-            const int32_t s = P.colsrc[gc];
-            const double z = (s >= 0) ? double(x[s]) : (s == kPadCol ? 0.0 : t[~s]);
-            gR[C.data_off + (int64_t(g) * m + lane) * W + w] = yb * z;
+            for (int w = 0; w < W; w += 2) reinterpret_cast<double2*>(d)[w / 2] = double2{yb * zg[w], yb * zg[w + 1]};
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) d[w] = yb * zg[w];
         }
     }
 }
@@ -319,6 +345,36 @@ __global__ __launch_bounds__(kThreads) void k_grad_r(DevPlan P, const RtItem* __
 struct GradPtrs {
     double* p[NFMT];
 };
+
+// dV(i, c) = x_i * s_e for the W columns of group `lane` of segment `it`:
+// row by row, so that a wave writes each row of the segment (ng W values)
+// as one contiguous run, each lane its W values together (16-byte stores
+// when W is even; the segment's offset and row length are multiples of W)
+template <int W, typename XT>
+__device__ inline void grad_v_group(const DevPlan& P, const VItem& it, const XT* __restrict__ x,
+                                    const double* __restrict__ s, double* __restrict__ gV, int lane) {
+    const ColPanel C = P.cp[it.q];
+    const int64_t ld = int64_t(it.ng) * W;
+    double sv[W];
+    const int64_t vc0 = C.vcol0[it.fmt] + int64_t(it.g0 + lane) * W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const int32_t e = P.vsrc[vc0 + w];
+        sv[w] = (e >= 0) ? s[e] : 0.0;
+    }
+    double* dst = gV + it.data_off + int64_t(lane) * W;
+    for (int i = 0; i < C.m; ++i) {
+        const double xi = double(x[C.col0 + i]);
+        double* d = dst + int64_t(i) * ld;
+        if constexpr (W % 2 == 0) {
+#pragma unroll
+            for (int w = 0; w < W; w += 2) reinterpret_cast<double2*>(d)[w / 2] = double2{xi * sv[w], xi * sv[w + 1]};
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) d[w] = xi * sv[w];
+        }
+    }
+}
 
 // dV(i, c) = x_i * s_e, one wave per item, lane = group
 template <typename XT>
@@ -333,16 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_v(DevPlan P, const XT* __rest
     for (int f = 0; f < NFMT; ++f)
         if (it.fmt == f) gV = gA.p[f];
     if (!gV || lane >= it.ng) return;
-    const ColPanel C = P.cp[it.q];
-    const int W = fmt_width(it.fmt);
-    const int64_t ld = int64_t(it.ng) * W;
-    for (int w = 0; w < W; ++w) {
-        const int64_t vc = C.vcol0[it.fmt] + int64_t(it.g0 + lane) * W + w;
-        const int32_t e = P.vsrc[vc];
-        const double sv = (e >= 0) ? s[e] : 0.0;
-        for (int i = 0; i < C.m; ++i)
-            gV[it.data_off + int64_t(i) * ld + int64_t(lane) * W + w] = double(x[C.col0 + i]) * sv;
-    }
+    HMVM_FMT_SWITCH(it.fmt, grad_v_group<Elem<T>::W>(P, it, x, s, gV, lane))
 }
 
 } // namespace
