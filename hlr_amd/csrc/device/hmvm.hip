@@ -353,18 +353,20 @@ struct GradPtrs {
 template <int W, typename XT>
 __device__ inline void grad_v_group(const DevPlan& P, const VItem& it, const XT* __restrict__ x,
                                     const double* __restrict__ s, double* __restrict__ gV, int lane) {
-    const ColPanel C = P.cp[it.q];
+    const ColPanel* C = P.cp + it.q;                     // read in place: vcol0 is indexed by the format
     const int64_t ld = int64_t(it.ng) * W;
     double sv[W];
-    const int64_t vc0 = C.vcol0[it.fmt] + int64_t(it.g0 + lane) * W;
+    const int64_t vc0 = C->vcol0[it.fmt] + int64_t(it.g0 + lane) * W;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const int32_t e = P.vsrc[vc0 + w];
         sv[w] = (e >= 0) ? s[e] : 0.0;
     }
     double* dst = gV + it.data_off + int64_t(lane) * W;
-    for (int i = 0; i < C.m; ++i) {
-        const double xi = double(x[C.col0 + i]);
+    const int m = C->m;
+    const int64_t col0 = C->col0;
+    for (int i = 0; i < m; ++i) {
+        const double xi = double(x[col0 + i]);
         double* d = dst + int64_t(i) * ld;
         if constexpr (W % 2 == 0) {
 #pragma unroll
