@@ -2,7 +2,7 @@
 
 `HMatVec.apply(x, plan, *arenas)` computes y = A x with A given by `plan`
 whose stored factors are the tensors `arenas` (one per storage format that
-the plan uses, in the plan's own layout, see Plan.arena_tensor).  The
+the plan uses, in the plan's own layout, see arena_tensor below).  The
 backward pass calls hmvm_grad: dL/dx = A^T dL/dy (transposed H-MVM) and
 dL/d(factors) = per-block outer products with V^T x recomputed.  Gradients
 with respect to factors are defined for FP64 and FP32 storage.

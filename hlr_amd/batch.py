@@ -12,7 +12,7 @@ from __future__ import annotations
 import ctypes as C
 
 from . import _native as N
-from .plan import Plan, _stream_handle
+from .plan import Plan, _stream_handle, _vec_dtype, check_vectors
 
 
 class Batch:
@@ -20,6 +20,9 @@ class Batch:
         self._b = None
         self.plans = list(plans)
         self.nrhs = [int(r) for r in nrhs]
+        if len(self.nrhs) != len(self.plans):
+            raise ValueError(f"batch: {len(self.plans)} plans but {len(self.nrhs)} right-hand-side counts")
+        self.vec_dtype = int(vec_dtype)
         arr = (C.c_void_p * len(plans))(*[p.handle for p in plans])
         rr = (C.c_int * len(plans))(*self.nrhs)
         b = C.c_void_p()
@@ -44,7 +47,27 @@ class Batch:
         return ptrs, lds
 
     def _args(self, alphas, xs, ys):
-        al = (C.c_double * len(xs))(*[float(a) for a in (alphas or [1.0] * len(xs))])
+        """Checked ctypes arrays of the members' alphas and vectors: one x, y
+        (and alpha) per member, each pair valid for its plan with exactly the
+        member's number of right-hand sides (the C side reads nplans entries
+        of every array and cannot check the tensors)."""
+        xs, ys = list(xs), list(ys)
+        n = len(self.plans)
+        if len(xs) != n or len(ys) != n:
+            raise ValueError(f"batch: {n} members, got {len(xs)} x and {len(ys)} y tensors")
+        alphas = [1.0] * n if alphas is None else list(alphas)
+        if len(alphas) != n:
+            raise ValueError(f"batch: {n} members, got {len(alphas)} alphas")
+        for i, (p, r, x, y) in enumerate(zip(self.plans, self.nrhs, xs, ys)):
+            check_vectors(p, N.OP_NORMAL, x, y, f"batch member {i}", nrhs=r, device=False)
+            if r > 1 and x.dim() != 2:
+                raise ValueError(f"batch member {i}: {r} right-hand sides need (nrhs, n) tensors")
+            if _vec_dtype(x) != self.vec_dtype:
+                raise ValueError(f"batch member {i}: vectors of {x.dtype}, the batch was created for "
+                                 f"{'float64' if self.vec_dtype == N.F64 else 'float32'}")
+        for i, (p, x, y) in enumerate(zip(self.plans, xs, ys)):
+            check_vectors(p, N.OP_NORMAL, x, y, f"batch member {i}")
+        al = (C.c_double * n)(*[float(a) for a in alphas])
         xp, lx = self._vecs(xs)
         yp, ly = self._vecs(ys)
         return al, xp, lx, yp, ly

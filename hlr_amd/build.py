@@ -69,7 +69,7 @@ def build_driver() -> Path:
     """mvm-hip: benchmark driver over the C++ mirror hlr::hip::mul_vec"""
     out = LIBDIR / "mvm-hip"
     tmp = LIBDIR / "mvm-hip.tmp"
-    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC / 'host'}", str(CSRC / "host" / "mvm_hip.cpp"),
+    _run([HIPCC, "-O2", "-std=c++20", f"-I{INCLUDE}", f"-I{CSRC / 'host'}", str(CSRC / "host" / "mvm_hip.cpp"),
           f"-L{LIBDIR}", "-lhmvm", "-lhlrgen", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
     os.replace(tmp, out)
     return out

@@ -156,7 +156,8 @@ struct hmvm_batch {
     std::vector<int> nrhs;
     std::vector<BatchMat> proto;            // per member, pointers to vectors filled at apply time
     BatchMat* h_mats = nullptr;             // pinned host staging of the descriptors
-    BatchMat* d_mats = nullptr;
+    BatchMat* d_mats = nullptr;             // descriptors of eager launches (apply, tune)
+    BatchMat* d_mats_graph = nullptr;       // the captured graph's own descriptors
     // per configuration: the units of all kinds back to back, kind k at
     // kind_ofs[c][k] .. kind_ofs[c][k + 1]
     std::vector<BatchUnit> units[kConfigs];
@@ -203,36 +204,41 @@ int fill_descriptors(hmvm_batch* b, const double* alphas, const void* const* xs,
 }
 
 template <typename XT, typename YT, int FAM>
-void launch_kinds(hmvm_batch* b, hipStream_t st) {
+void launch_kinds(hmvm_batch* b, const BatchMat* mats, hipStream_t st) {
     const int c = b->config;
     const BatchUnit* u = b->d_units[c];
     const int64_t* o = b->kind_ofs[c];
     auto grid = [&](int k) { return unsigned(o[k + 1] - o[k]); };
-    if (grid(K_XT)) k_bxt<XT><<<grid(K_XT), kThreads, b->lds[K_XT], st>>>(b->d_mats, u + o[K_XT]);
-    if (grid(K_V1)) k_bv1<XT, FAM><<<grid(K_V1), kThreads, b->lds[K_V1], st>>>(b->d_mats, u + o[K_V1]);
-    if (grid(K_VM1)) k_bvm<1, FAM><<<grid(K_VM1), kThreads, b->lds[K_VM1], st>>>(b->d_mats, u + o[K_VM1]);
-    if (grid(K_VM2)) k_bvm<2, FAM><<<grid(K_VM2), kThreads, b->lds[K_VM2], st>>>(b->d_mats, u + o[K_VM2]);
-    if (grid(K_VM3)) k_bvm<3, FAM><<<grid(K_VM3), kThreads, b->lds[K_VM3], st>>>(b->d_mats, u + o[K_VM3]);
-    if (grid(K_VM4)) k_bvm<4, FAM><<<grid(K_VM4), kThreads, b->lds[K_VM4], st>>>(b->d_mats, u + o[K_VM4]);
-    if (grid(K_PS)) k_bps<<<grid(K_PS), kThreads, 0, st>>>(b->d_mats, u + o[K_PS]);
-    if (grid(K_R1)) k_br1<XT, YT, FAM><<<grid(K_R1), kThreads, b->lds[K_R1], st>>>(b->d_mats, u + o[K_R1]);
-    if (grid(K_RM1)) k_brm<1, YT, FAM><<<grid(K_RM1), kThreads, b->lds[K_RM1], st>>>(b->d_mats, u + o[K_RM1]);
-    if (grid(K_RM2)) k_brm<2, YT, FAM><<<grid(K_RM2), kThreads, b->lds[K_RM2], st>>>(b->d_mats, u + o[K_RM2]);
-    if (grid(K_RM3)) k_brm<3, YT, FAM><<<grid(K_RM3), kThreads, b->lds[K_RM3], st>>>(b->d_mats, u + o[K_RM3]);
-    if (grid(K_RM4)) k_brm<4, YT, FAM><<<grid(K_RM4), kThreads, b->lds[K_RM4], st>>>(b->d_mats, u + o[K_RM4]);
+    if (grid(K_XT)) k_bxt<XT><<<grid(K_XT), kThreads, b->lds[K_XT], st>>>(mats, u + o[K_XT]);
+    if (grid(K_V1)) k_bv1<XT, FAM><<<grid(K_V1), kThreads, b->lds[K_V1], st>>>(mats, u + o[K_V1]);
+    if (grid(K_VM1)) k_bvm<1, FAM><<<grid(K_VM1), kThreads, b->lds[K_VM1], st>>>(mats, u + o[K_VM1]);
+    if (grid(K_VM2)) k_bvm<2, FAM><<<grid(K_VM2), kThreads, b->lds[K_VM2], st>>>(mats, u + o[K_VM2]);
+    if (grid(K_VM3)) k_bvm<3, FAM><<<grid(K_VM3), kThreads, b->lds[K_VM3], st>>>(mats, u + o[K_VM3]);
+    if (grid(K_VM4)) k_bvm<4, FAM><<<grid(K_VM4), kThreads, b->lds[K_VM4], st>>>(mats, u + o[K_VM4]);
+    if (grid(K_PS)) k_bps<<<grid(K_PS), kThreads, 0, st>>>(mats, u + o[K_PS]);
+    if (grid(K_R1)) k_br1<XT, YT, FAM><<<grid(K_R1), kThreads, b->lds[K_R1], st>>>(mats, u + o[K_R1]);
+    if (grid(K_RM1)) k_brm<1, YT, FAM><<<grid(K_RM1), kThreads, b->lds[K_RM1], st>>>(mats, u + o[K_RM1]);
+    if (grid(K_RM2)) k_brm<2, YT, FAM><<<grid(K_RM2), kThreads, b->lds[K_RM2], st>>>(mats, u + o[K_RM2]);
+    if (grid(K_RM3)) k_brm<3, YT, FAM><<<grid(K_RM3), kThreads, b->lds[K_RM3], st>>>(mats, u + o[K_RM3]);
+    if (grid(K_RM4)) k_brm<4, YT, FAM><<<grid(K_RM4), kThreads, b->lds[K_RM4], st>>>(mats, u + o[K_RM4]);
 }
 
-int launch(hmvm_batch* b, hipStream_t st) {
-    if (hipMemcpyAsync(b->d_mats, b->h_mats, sizeof(BatchMat) * size_t(b->nmat), hipMemcpyHostToDevice, st) !=
-        hipSuccess)
+// Eager launches (mats == b->d_mats) upload the pinned descriptors first, a
+// copy on the stream; a captured graph (mats == b->d_mats_graph) takes
+// descriptors uploaded before its capture, so that no node of the graph
+// reads the shared staging buffer, which later calls rewrite.
+int launch(hmvm_batch* b, BatchMat* mats, hipStream_t st) {
+    if (mats == b->d_mats &&
+        hipMemcpyAsync(b->d_mats, b->h_mats, sizeof(BatchMat) * size_t(b->nmat), hipMemcpyHostToDevice, st) !=
+            hipSuccess)
         return HMVM_ERR_DEVICE;
     // the matrix-core kernels come in one instantiation per format family
     auto go = [&](auto fam) {
         constexpr int FAM = decltype(fam)::value;
         if (b->vdt == HMVM_F64)
-            launch_kinds<double, double, FAM>(b, st);
+            launch_kinds<double, double, FAM>(b, mats, st);
         else
-            launch_kinds<float, float, FAM>(b, st);
+            launch_kinds<float, float, FAM>(b, mats, st);
     };
     switch (b->family) {
     case kFamHw: go(std::integral_constant<int, kFamHw>()); break;
@@ -369,6 +375,7 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
 
     int rc;
     if ((rc = dev_alloc(b.get(), &b->d_mats, size_t(nplans))) != HMVM_OK) { release(b.get()); return rc; }
+    if ((rc = dev_alloc(b.get(), &b->d_mats_graph, size_t(nplans))) != HMVM_OK) { release(b.get()); return rc; }
     for (int c = 0; c < kConfigs; ++c)
         if ((rc = dev_alloc(b.get(), &b->d_units[c], b->units[c].size())) != HMVM_OK) { release(b.get()); return rc; }
     if ((rc = dev_alloc(b.get(), &b->d_ws, size_t(wsz))) != HMVM_OK) { release(b.get()); return rc; }
@@ -417,7 +424,7 @@ int hmvm_batch_apply(hmvm_batch* b, const double* alphas, const void* const* xs,
     if (hipStreamSynchronize(st) != hipSuccess) return HMVM_ERR_DEVICE;
     const int rc = fill_descriptors(b, alphas, xs, ldx, ys, ldy);
     if (rc != HMVM_OK) return rc;
-    return launch(b, st);
+    return launch(b, b->d_mats, st);
 }
 
 int hmvm_batch_capture(hmvm_batch* b, const double* alphas, const void* const* xs, const int64_t* ldx,
@@ -431,8 +438,14 @@ int hmvm_batch_capture(hmvm_batch* b, const double* alphas, const void* const* x
     if (rc != HMVM_OK) return rc;
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    // the graph's descriptors: uploaded once, synchronously, after every
+    // earlier replay (on any stream) has finished reading them
+    if (hipDeviceSynchronize() != hipSuccess) return HMVM_ERR_DEVICE;
+    if (hipMemcpy(b->d_mats_graph, b->h_mats, sizeof(BatchMat) * size_t(b->nmat), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return HMVM_ERR_DEVICE;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return HMVM_ERR_DEVICE;
-    rc = launch(b, st);
+    rc = launch(b, b->d_mats_graph, st);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc != HMVM_OK || e != hipSuccess) {
@@ -480,11 +493,11 @@ int hmvm_batch_tune(hmvm_batch* b, const void* const* xs, const int64_t* ldx, vo
     int best_cfg = -1;
     for (int c = 0; c < kConfigs && rc == HMVM_OK; ++c) {
         b->config = c;
-        if ((rc = launch(b, st)) != HMVM_OK) break;          // warm-up
+        if ((rc = launch(b, b->d_mats, st)) != HMVM_OK) break;   // warm-up
         float tot = 0;
         for (int rep = 0; rep < 3 && rc == HMVM_OK; ++rep) {
             if (hipEventRecord(e0, st) != hipSuccess) { rc = HMVM_ERR_DEVICE; break; }
-            rc = launch(b, st);
+            rc = launch(b, b->d_mats, st);
             if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                 rc = HMVM_ERR_DEVICE;
                 break;

@@ -823,6 +823,12 @@ int hmvm_grad(const hmvm_plan* p, const void* x, const void* ybar, void* gx, voi
               void* stream) {
     int rc = check_apply_args(p, HMVM_OP_NORMAL, x, ybar, 1, vdt);
     if (rc != HMVM_OK) return rc;
+    // gx is read and written as vector elements; the gradient kernels store
+    // 16 bytes (two doubles) at a time into the arenas
+    if (reinterpret_cast<uintptr_t>(gx) % ((vdt == HMVM_F64) ? 8 : 4)) return HMVM_ERR_LAYOUT;
+    if (grad_arena)
+        for (int f = 0; f < NFMT; ++f)
+            if (reinterpret_cast<uintptr_t>(grad_arena[f]) % 16) return HMVM_ERR_LAYOUT;
     DeviceGuard g(p->device);
     if (!g.ok) return HMVM_ERR_DEVICE;
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -244,24 +244,28 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     L.blocks.assign(blocks, blocks + nb);
 
     // ---- validate leaves (cf. HLR_ASSERT in apply_add, check_data()) ----
+    // [off, off + len) inside [0, total), compared in subtract form: offsets
+    // come from files and may be anywhere in int64, so off + len could wrap
+    auto inside = [](int64_t off, int64_t len, int64_t total) {
+        return off >= 0 && len >= 0 && total >= 0 && len <= total && off <= total - len;
+    };
+    if (nrows < 0 || ncols < 0 || arena_len < 0) return HMVM_ERR_INVALID_ARG;
     for (int64_t i = 0; i < nb; ++i) {
         const hmvm_block& b = blocks[i];
         if (b.kind < HMVM_BLOCK_DENSE || b.kind > HMVM_BLOCK_LOWRANK_SV) return HMVM_ERR_BAD_BLOCK;
-        if (b.nrows < 0 || b.ncols < 0 || b.row_ofs < 0 || b.col_ofs < 0) return HMVM_ERR_BAD_BLOCK;
-        if (b.row_ofs + b.nrows > nrows || b.col_ofs + b.ncols > ncols) return HMVM_ERR_BAD_BLOCK;
-        const int64_t m = b.nrows, n = b.ncols;
+        if (!inside(b.row_ofs, b.nrows, nrows) || !inside(b.col_ofs, b.ncols, ncols)) return HMVM_ERR_BAD_BLOCK;
+        const int64_t m = b.nrows, n = b.ncols;   // < 2^31 each: m * n and m * k cannot overflow
         if (b.kind == HMVM_BLOCK_DENSE) {
-            if (m * n > 0 && (b.off_a < 0 || b.off_a + m * n > arena_len)) return HMVM_ERR_BAD_BLOCK;
+            if (m * n > 0 && !inside(b.off_a, m * n, arena_len)) return HMVM_ERR_BAD_BLOCK;
         } else {
             const int64_t k = b.rank;
             if (k < 0 || k > std::min(m, n)) return HMVM_ERR_BAD_BLOCK;
             if (k > 0) {
-                if (b.off_a < 0 || b.off_a + m * k > arena_len) return HMVM_ERR_BAD_BLOCK;
-                if (b.off_b < 0 || b.off_b + n * k > arena_len) return HMVM_ERR_BAD_BLOCK;
+                if (!inside(b.off_a, m * k, arena_len)) return HMVM_ERR_BAD_BLOCK;
+                if (!inside(b.off_b, n * k, arena_len)) return HMVM_ERR_BAD_BLOCK;
             }
-            if (b.kind == HMVM_BLOCK_LOWRANK_SV && k > 0 && (b.off_s < 0 || b.off_s + k > arena_len))
-                return HMVM_ERR_BAD_BLOCK;
-            if (b.off_s >= 0 && k > 0 && b.off_s + k > arena_len) return HMVM_ERR_BAD_BLOCK;
+            if (b.kind == HMVM_BLOCK_LOWRANK_SV && k > 0 && !inside(b.off_s, k, arena_len)) return HMVM_ERR_BAD_BLOCK;
+            if (b.off_s >= 0 && k > 0 && !inside(b.off_s, k, arena_len)) return HMVM_ERR_BAD_BLOCK;
         }
     }
     if (opts.check_finite) {

@@ -28,6 +28,13 @@
 #include "hmvm.h"
 
 namespace hlr {
+
+// HLR's arithmetic trait (arith/defaults.hh:20-24: primary template false,
+// `arithmetic_type` is the concept is_arithmetic_v<T>).  Declared here, not
+// defined, so that hip_arithmetic registers with it below whether HLR's
+// headers are included before this one, after it, or not at all.
+template <typename T> struct is_arithmetic;
+
 namespace hip {
 
 enum matop_t { apply_normal = HMVM_OP_NORMAL, apply_transposed = HMVM_OP_TRANSPOSED,
@@ -99,6 +106,8 @@ public:
     hmatrix& operator=(const hmatrix&) = delete;
     ~hmatrix() { if (p_) hmvm_plan_destroy(p_); }
 
+    using value_t = double;       // factors enter as FP64 (the plan may store them narrower)
+
     const hmvm_plan* plan() const { return p_; }
     int64_t nrows() const { return info_.nrows; }
     int64_t ncols() const { return info_.ncols; }
@@ -165,10 +174,20 @@ void mul_vec(const value_t alpha, const matop_t op, const hmatrix& M, const devi
 
 } // namespace tlr
 
+// sizes of an operator, as generic algorithms ask for them (nrows(A),
+// ncols(A) in arith/detail/norm.hh:458-459), found by argument lookup
+inline size_t nrows(const hmatrix& M) { return size_t(M.nrows()); }
+inline size_t ncols(const hmatrix& M) { return size_t(M.ncols()); }
+
 // arithmetic object for generic algorithms (norm::spectral and friends call
-// arithmetic.prod(alpha, op, A, x, y), arith/detail/norm.hh:492-494)
+// arithmetic.prod(alpha, op, A, x, y), arith/detail/norm.hh:492-494).
+// Registered with hlr::is_arithmetic below, so it satisfies HLR's
+// `arithmetic_type` concept.  Two kinds of vectors:
+//   - device_vector: stream-ordered on `stream`, as hlr::hip::mul_vec;
+//   - host vectors with data() and length() (HLR's blas::vector and
+//     scalar_vector, std::vector through host_vector_ref): staged through
+//     device memory and synchronous, as the reference's call is.
 struct hip_arithmetic {
-    static constexpr bool is_arithmetic = true;
     hipStream_t stream = nullptr;
 
     template <typename value_t>
@@ -182,6 +201,38 @@ struct hip_arithmetic {
               device_vector<value_t>& y) const {
         hlr::hip::mul_vec(alpha, op, M, x, y, stream);
     }
+
+    template <typename value_t, typename host_vector_t>
+    void mul_vec(const value_t alpha, const matop_t op, const hmatrix& M, const host_vector_t& x,
+                 host_vector_t& y) const {
+        if (alpha == value_t(0)) return;                     // tbb/arith.hh:54-55: y untouched
+        const size_t nx = size_t(x.length()), ny = size_t(y.length());
+        if (int64_t(nx) != M.ncols(op) || int64_t(ny) != M.nrows(op))
+            throw std::runtime_error("prod: vector sizes do not match the matrix");
+        device_vector<value_t> dx(nx), dy(ny);
+        check_hip(hipMemcpyAsync(dx.data(), x.data(), sizeof(value_t) * nx, hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync");
+        check_hip(hipMemcpyAsync(dy.data(), y.data(), sizeof(value_t) * ny, hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync");
+        hlr::hip::mul_vec(alpha, op, M, dx, dy, stream);
+        check_hip(hipMemcpyAsync(y.data(), dy.data(), sizeof(value_t) * ny, hipMemcpyDeviceToHost, stream),
+                  "hipMemcpyAsync");
+        check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    template <typename value_t, typename host_vector_t>
+    void prod(const value_t alpha, const matop_t op, const hmatrix& M, const host_vector_t& x,
+              host_vector_t& y) const {
+        mul_vec(alpha, op, M, x, y);
+    }
+};
+
+// a std::vector seen through the data() / length() interface of HLR's vectors
+template <typename value_t>
+struct host_vector_ref {
+    std::vector<value_t>& v;
+    value_t* data() const { return v.data(); }
+    size_t length() const { return v.size(); }
 };
 
 namespace norm {
@@ -229,4 +280,12 @@ inline int64_t linear_iteration(const hmatrix& A, const hmatrix& W, const device
 }
 
 } // namespace hip
+
+// registration with HLR's trait, for the value and reference forms generic
+// algorithms deduce (as arith/defaults.hh:101-104 does for default_arithmetic)
+template <> struct is_arithmetic<hip::hip_arithmetic> { static constexpr bool value = true; };
+template <> struct is_arithmetic<const hip::hip_arithmetic> { static constexpr bool value = true; };
+template <> struct is_arithmetic<hip::hip_arithmetic&> { static constexpr bool value = true; };
+template <> struct is_arithmetic<const hip::hip_arithmetic&> { static constexpr bool value = true; };
+
 } // namespace hlr

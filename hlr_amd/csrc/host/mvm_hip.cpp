@@ -10,10 +10,13 @@
 // usage: mvm-hip [--app log|logkernel|materncov|invr] [-n N] [--ntile K]
 //                [--eta E] [--eps E] [--nbench B] [--prec fp64|fp32|valr|bfp]
 //                [--tlr] [--xrand] [--op normal|transposed]
-//                [--save FILE | --load FILE]
+//                [--save FILE | --load FILE] [--norm]
 // --save stores the generated matrix as an H-matrix file; --load reads one
 // instead of generating (the problem options then only label the output,
 // except --eps, which sets the plan's tolerance).
+// --norm: the spectral norm twice, once by a generic power iteration written
+// against HLR's arithmetic concept and run with hip_arithmetic on host
+// vectors (the plugin path), once by hlr::hip::norm::spectral on the device.
 
 #include <algorithm>
 #include <chrono>
@@ -21,20 +24,73 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "hlr_hip.hh"
 #include "hlrgen.h"
 
+// HLR's arithmetic trait and concept (arith/defaults.hh:20-24), defined after
+// hlr_hip.hh has declared the trait and registered hip_arithmetic with it
+namespace hlr {
+template <typename T> struct is_arithmetic { static constexpr bool value = false; };
+template <typename T> inline constexpr bool is_arithmetic_v = is_arithmetic<T>::value;
+template <typename T> concept arithmetic_type = is_arithmetic_v<T>;
+}
+
 using namespace hlr::hip;
+
+static_assert(hlr::arithmetic_type<hip_arithmetic> && hlr::arithmetic_type<const hip_arithmetic&>,
+              "hip_arithmetic satisfies HLR's arithmetic concept");
+
+namespace {
+
+// A generic algorithm in the shape of HLR's norm::spectral
+// (arith/detail/norm.hh:448-527): constrained by arithmetic_type, it sees
+// the operator only through nrows/ncols and arithmetic.prod on host vectors.
+template <hlr::arithmetic_type arithmetic_t, typename operator_t>
+double generic_spectral(arithmetic_t&& arithmetic, const operator_t& A, const double atol, const size_t amax_it) {
+    const size_t nr = nrows(A), nc = ncols(A);
+    std::vector<double> xv(nc), yv(nc), tv(nr);
+    host_vector_ref<double> x{xv}, y{yv}, t{tv};
+    auto gen = std::default_random_engine();
+    auto dist = std::uniform_real_distribution<double>(-1.0, 1.0);
+    for (auto& v : xv) v = dist(gen);
+    auto norm2 = [](const std::vector<double>& v) { double s = 0; for (double e : v) s += e * e; return std::sqrt(s); };
+    const double nx = norm2(xv);
+    for (auto& v : xv) v /= nx;
+    const size_t max_it = amax_it == 0 ? std::max(size_t(5), std::min(nr, nc) / 10) : amax_it;
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double tol = atol == 0 ? std::sqrt(eps) : atol, abs_tol = std::min(10 * eps, tol), zero = eps * eps;
+    double lambda = 1.0;
+    for (size_t i = 0; i < max_it; ++i) {
+        std::fill(tv.begin(), tv.end(), 0.0);
+        arithmetic.prod(1.0, apply_normal, A, x, t);
+        std::fill(yv.begin(), yv.end(), 0.0);
+        arithmetic.prod(1.0, apply_adjoint, A, t, y);
+        double dot = 0;
+        for (size_t j = 0; j < nc; ++j) dot += xv[j] * yv[j];
+        const double lambda_new = std::sqrt(std::fabs(dot)), norm_y = norm2(yv);
+        if (std::fabs((lambda_new - lambda) / lambda) < tol) return lambda_new;
+        if (i > 5 && std::fabs(lambda_new - lambda) < abs_tol) return lambda_new;
+        if (lambda_new < zero) return lambda_new;
+        lambda = lambda_new;
+        if (norm_y <= zero) break;
+        for (size_t j = 0; j < nc; ++j) xv[j] = yv[j] / norm_y;
+    }
+    return lambda;
+}
+
+} // namespace
 
 int main(int argc, char** argv) {
     hlrgen_config cfg;
     hlrgen_default_config(&cfg);
     std::string app = "log", prec = "fp64", opname = "normal", save, load;
     int nbench = 5;
-    bool xrand = false;
+    bool xrand = false, norm_check = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -53,6 +109,7 @@ int main(int argc, char** argv) {
         else if (a == "--xrand") xrand = true;
         else if (a == "--save") save = next();
         else if (a == "--load") load = next();
+        else if (a == "--norm") norm_check = true;
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     if (app == "log") { cfg.kernel = HLRGEN_KERNEL_LOG; cfg.geometry = HLRGEN_GEOM_LINE; }
@@ -99,6 +156,15 @@ int main(int argc, char** argv) {
         hmatrix A(blocks, arena, n, n, &opts, 0);
         std::printf("plan      : %s, %.3f GB of factors, %lld dense / %lld low-rank leaves\n", prec.c_str(),
                     A.info().factor_bytes * 1e-9, (long long)A.info().ndense, (long long)A.info().nlowrank);
+
+        if (norm_check) {
+            hip_arithmetic arith;
+            const double gen = generic_spectral(arith, A, 1e-10, 40);
+            const double dev = hlr::hip::norm::spectral(arith, A, 1e-10, 40);
+            std::printf("norm      : generic %.15e device %.15e rel.diff %.3e\n", gen, dev,
+                        std::fabs(gen - dev) / dev);
+            return 0;
+        }
 
         const int nmvm = 50;
         std::vector<double> xh(n, 1.0);

@@ -35,6 +35,59 @@ def _vec_dtype(t):
     raise TypeError(f"unsupported vector dtype {t.dtype}")
 
 
+def _on_device(t, device: int, what: str) -> None:
+    if t.device.type != "cuda" or t.device.index != device:
+        raise ValueError(f"{what}: tensor on {t.device}, the plan lives on HIP device {device}")
+
+
+def check_vectors(plan, op: int, x, y, what: str = "apply", nrhs: int | None = None, device: bool = True):
+    """Validate the vectors of y += alpha op(A) x before any C call (the
+    C-ABI takes raw pointers and cannot check sizes itself).
+
+    x and y: torch tensors of one floating dtype (float64 or float32), either
+    1-D of ncols(op) / nrows(op) entries with unit stride, or 2-D (nrhs, n)
+    with unit stride along n and row stride >= n (column-major n x nrhs), on
+    the plan's device (checked last, and only when `device` is true).  nrhs,
+    when given, is the count the caller expects.  Returns (nrhs, ldx, ldy);
+    raises TypeError / ValueError."""
+    import torch
+    for t, name in ((x, "x"), (y, "y")):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{what}: {name} is not a torch tensor")
+    _vec_dtype(x)
+    if y.dtype != x.dtype:
+        raise ValueError(f"{what}: x is {x.dtype} but y is {y.dtype}")
+    trans = op in (N.OP_TRANSPOSED, N.OP_ADJOINT)
+    nin = plan.nrows if trans else plan.ncols
+    nout = plan.ncols if trans else plan.nrows
+    if x.dim() != y.dim() or x.dim() not in (1, 2):
+        raise ValueError(f"{what}: x and y must both be 1-D or both (nrhs, n)")
+    if x.dim() == 1:
+        if x.numel() != nin or y.numel() != nout:
+            raise ValueError(f"{what}: expected x of {nin} and y of {nout} entries, got {x.numel()} and {y.numel()}")
+        if (nin > 1 and x.stride(0) != 1) or (nout > 1 and y.stride(0) != 1):
+            raise ValueError(f"{what}: vectors must have unit stride")
+        r, ldx, ldy = 1, 0, 0
+    else:
+        r = x.shape[0]
+        if r < 1 or y.shape[0] != r:
+            raise ValueError(f"{what}: x and y must hold the same number (>= 1) of right-hand sides")
+        if x.shape[1] != nin or y.shape[1] != nout:
+            raise ValueError(f"{what}: expected (nrhs, {nin}) and (nrhs, {nout}), got {tuple(x.shape)} and "
+                             f"{tuple(y.shape)}")
+        if (nin > 1 and x.stride(1) != 1) or (nout > 1 and y.stride(1) != 1):
+            raise ValueError(f"{what}: x, y must be (nrhs, n) tensors with unit stride along n")
+        ldx, ldy = x.stride(0), y.stride(0)
+        if r > 1 and (ldx < nin or ldy < nout):
+            raise ValueError(f"{what}: row stride below the vector length")
+    if nrhs is not None and r != nrhs:
+        raise ValueError(f"{what}: expected {nrhs} right-hand sides, got {r}")
+    if device:
+        _on_device(x, plan.device, what)
+        _on_device(y, plan.device, what)
+    return r, ldx, ldy
+
+
 class Plan:
     """hmvm_plan wrapper.
 
@@ -108,13 +161,8 @@ class Plan:
     def apply(self, alpha: float, op: int, x, y, stream=None) -> None:
         """y += alpha op(A) x for torch tensors (1-D, or 2-D column-major
         n x nrhs, i.e. a transposed contiguous nrhs x n tensor)."""
-        nrhs, ldx, ldy = 1, 0, 0
-        if x.dim() == 2:
-            # column-major n x nrhs == row-major nrhs x n
-            nrhs = x.shape[0]
-            ldx, ldy = x.stride(0), y.stride(0)
-            if x.stride(1) != 1 or y.stride(1) != 1 or y.shape[0] != nrhs:
-                raise ValueError("x, y must be (nrhs, n) tensors with unit stride along n")
+        # column-major n x nrhs == row-major nrhs x n
+        nrhs, ldx, ldy = check_vectors(self, op, x, y, "apply")
         rc = N.hmvm().hmvm_apply(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()), ldx,
                                  C.c_void_p(y.data_ptr()), ldy, nrhs, _vec_dtype(x), _stream_handle(stream))
         N.check(rc, "hmvm_apply")
@@ -122,6 +170,9 @@ class Plan:
     def apply_timed(self, alpha: float, op: int, x, y, stream=None):
         """apply with HIP events around the kernel phases; returns
         (reduction pass, partial sums, output pass) durations in ms."""
+        if x.dim() != 1:
+            raise ValueError("apply_timed: one right-hand side, 1-D vectors")
+        check_vectors(self, op, x, y, "apply_timed", nrhs=1)
         ms = (C.c_float * 3)()
         rc = N.hmvm().hmvm_apply_timed(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()),
                                        C.c_void_p(y.data_ptr()), _vec_dtype(x), _stream_handle(stream), ms)
@@ -129,9 +180,10 @@ class Plan:
         return tuple(float(v) for v in ms)
 
     def apply_ws(self, alpha, op, x, y, ws, stream=None) -> None:
-        nrhs = x.shape[0] if x.dim() == 2 else 1
-        ldx = x.stride(0) if x.dim() == 2 else 0
-        ldy = y.stride(0) if y.dim() == 2 else 0
+        """apply with a caller workspace `ws` (a tensor on the plan's device
+        of at least workspace_bytes(op, nrhs) bytes)."""
+        nrhs, ldx, ldy = check_vectors(self, op, x, y, "apply_ws")
+        _check_buffer(ws, self.device, self.workspace_bytes(op, nrhs), "apply_ws: workspace")
         rc = N.hmvm().hmvm_apply_ws(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()), ldx,
                                     C.c_void_p(y.data_ptr()), ldy, nrhs, _vec_dtype(x), C.c_void_p(ws.data_ptr()),
                                     ws.numel() * ws.element_size(), _stream_handle(stream))
@@ -143,6 +195,9 @@ class Plan:
         return int(b.value)
 
     def tlr_apply(self, alpha: float, op: int, x, y, stream=None) -> None:
+        if x.dim() != 1:
+            raise ValueError("tlr_apply: one right-hand side, 1-D vectors")
+        check_vectors(self, op, x, y, "tlr_apply", nrhs=1)
         rc = N.hmvm().hmvm_tlr_apply(self._p, float(alpha), int(op), C.c_void_p(x.data_ptr()),
                                      C.c_void_p(y.data_ptr()), _vec_dtype(x), _stream_handle(stream))
         N.check(rc, "hmvm_tlr_apply")
@@ -186,6 +241,9 @@ class Plan:
         return out
 
     def set_arena(self, fmt: int, src) -> None:
+        """Replace the stored arena of format fmt by the device tensor src
+        (at least arena_bytes(fmt) bytes, in the plan's layout)."""
+        _check_buffer(src, self.device, self.arena_bytes(fmt), "set_arena")
         N.check(N.hmvm().hmvm_plan_set_arena(self._p, fmt, C.c_void_p(src.data_ptr()), _stream_handle(None)),
                 "set_arena")
 
@@ -208,8 +266,19 @@ class Plan:
 
     def grad(self, x, ybar, gx=None, factors: bool = True, stream=None):
         """Adjoint of y = A x: gx += A^T ybar, and per-format FP64 gradient
-        arenas (dL/d stored factor value, in the plan's layout)."""
+        arenas in the plan's layout.  For FP64, FP32 and BF16 storage an
+        entry is dL/d(stored value); for the scaled formats (FP8 and block
+        floating point) it is dL/d(dequantised value), i.e. with respect to
+        the stored value times its panel-column scale."""
         import torch
+        if x.dim() != 1 or ybar.dim() != 1:
+            raise ValueError("grad: x and ybar must be 1-D")
+        if gx is not None and (not isinstance(gx, torch.Tensor) or gx.dim() != 1 or gx.numel() != self.ncols
+                               or gx.dtype != x.dtype or (self.ncols > 1 and gx.stride(0) != 1)):
+            raise ValueError(f"grad: gx must be a unit-stride {x.dtype} vector of {self.ncols} entries")
+        check_vectors(self, N.OP_NORMAL, x, ybar, "grad")
+        if gx is not None:
+            _on_device(gx, self.device, "grad (gx)")
         garena = [None] * N.NFMT
         ptrs = (C.c_void_p * N.NFMT)()
         if factors:
@@ -223,6 +292,15 @@ class Plan:
                                 ptrs if factors else None, _vec_dtype(x), _stream_handle(stream))
         N.check(rc, "hmvm_grad")
         return garena
+
+
+def _check_buffer(t, device: int, nbytes: int, what: str) -> None:
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous torch tensor")
+    if t.numel() * t.element_size() < nbytes:
+        raise ValueError(f"{what}: {t.numel() * t.element_size()} bytes, {nbytes} needed")
+    _on_device(t, device, what)
 
 
 def _f64_vector(t, n: int, what: str):

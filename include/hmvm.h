@@ -223,8 +223,13 @@ int hmvm_tlr_apply(const hmvm_plan* plan, double alpha, int op,
  *   dU_b    = ybar_b (V_b^T x_b)^T              (V^T x recomputed, not stored)
  *   dV_b    = x_b (U_b^T ybar_b)^T
  *   dD_b    = ybar_b x_b^T
- * Gradients are written into device buffers in the plan's own arena layout
- * (one per storage format, see hmvm_plan_arena_elems); pass NULL to skip.
+ * Gradients are written into device buffers of FP64 values in the plan's own
+ * arena layout (one per storage format, hmvm_plan_arena_elems values each);
+ * pass NULL to skip.  Each buffer must be 16-byte aligned and gx aligned to
+ * its element size (HMVM_ERR_LAYOUT otherwise).  For FP64, FP32 and BF16
+ * storage an entry is the gradient with respect to the stored value; for
+ * the scaled formats (FP8, block floating point) it is the gradient with
+ * respect to the dequantised value (stored value times its column scale).
  * No counterpart in the reference (it has no differentiation); the transposed
  * product follows arith/blas.hh:910-948.
  */
@@ -274,7 +279,13 @@ int hmvm_batch_apply(hmvm_batch* batch, const double* alphas,
                      const void* const* xs, const int64_t* ldx,
                      void* const* ys, const int64_t* ldy, void* stream);
 
-/* capture hmvm_batch_apply into a HIP graph (pointers fixed), replay it */
+/*
+ * Capture hmvm_batch_apply into a HIP graph, replay it.  The graph keeps the
+ * alphas, vector pointers and leading dimensions given at capture (in a
+ * descriptor buffer of its own), whatever later apply, tune or capture calls
+ * pass; it runs the schedule chosen at capture.  Applies and replays of one
+ * batch share its workspace: calls on different streams must not overlap.
+ */
 int hmvm_batch_capture(hmvm_batch* batch, const double* alphas,
                        const void* const* xs, const int64_t* ldx,
                        void* const* ys, const int64_t* ldy, void* stream);

@@ -122,3 +122,39 @@ def test_batch_block_floating_point(cuda, members):
     b.check()
     _check(hms, nrhs, xs, ys, [1.0] * len(hms), 1e-6)
     b.close()
+
+
+def test_batch_graph_keeps_its_vectors(cuda, members):
+    """A captured graph replays the product it was captured with: after an
+    eager apply with other vectors and alphas, and after the autotuner, a
+    replay writes the captured y (checked against the oracle) and leaves the
+    other call's y untouched."""
+    import torch
+    hms, nrhs = members
+    plans = [Plan.from_hmatrix(hm, precision="fp64") for hm in hms]
+    b = Batch(plans, nrhs)
+    xs1, ys1 = _vectors(hms, nrhs, cuda, 21)
+    xs2, ys2 = _vectors(hms, nrhs, cuda, 22)
+    a1 = [0.25 + i for i in range(len(hms))]
+    a2 = [-3.0 - i for i in range(len(hms))]
+    print("[gpu] batch capture, then apply and tune with other vectors, then replay", flush=True)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        b.capture(xs1, ys1, a1, stream=stream)
+    stream.synchronize()
+    b.apply(xs2, ys2, a2)
+    b.check()
+    _check(hms, nrhs, xs2, ys2, a2, 1e-13)
+    scratch = [torch.zeros_like(y) for y in ys2]
+    b.tune(xs2, scratch)
+    torch.cuda.synchronize()
+    before = [y.clone() for y in ys2]
+    for y in ys1:
+        y.zero_()
+    torch.cuda.synchronize()
+    b.replay(stream=stream)
+    b.check(stream=stream)
+    _check(hms, nrhs, xs1, ys1, a1, 1e-13)
+    for y, y0 in zip(ys2, before):
+        assert torch.equal(y, y0)
+    b.close()

@@ -28,3 +28,16 @@ def test_driver(args, tol):
     m = re.search(r"error\s+=\s+([0-9.eE+-]+)", out.stdout)
     assert m, out.stdout
     assert float(m.group(1)) <= tol, out.stdout
+
+
+def test_arithmetic_plugin_generic_algorithm():
+    """hip_arithmetic as HLR's arithmetic plugin: a generic power iteration
+    constrained by the arithmetic concept (norm::spectral's shape) runs with
+    it on host vectors and agrees with the device spectral norm."""
+    print("[gpu] mvm-hip --norm", flush=True)
+    out = subprocess.run([str(DRIVER), "--app", "logkernel", "-n", "4096", "--norm"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    m = re.search(r"generic\s+([0-9.eE+-]+)\s+device\s+([0-9.eE+-]+)\s+rel\.diff\s+([0-9.eE+-]+)", out.stdout)
+    assert m, out.stdout
+    assert float(m.group(1)) > 0 and float(m.group(3)) <= 1e-10, out.stdout

@@ -173,3 +173,28 @@ def test_plan_from_file_equals_plan_from_memory(tmp_path, prec):
     # the Python spelling reads the same matrix
     q = Plan.from_file(path, precision=prec, eps=1e-6, device=-1)
     assert q.info() == ref.info()
+
+
+def test_offsets_near_int64_max_rejected(logm, tmp_path):
+    """A record whose offset would wrap int64 when added to its length is
+    rejected by the plan build (HMVM_ERR_BAD_BLOCK), also when the file's
+    checksums are recomputed so that the reader accepts it."""
+    good = tmp_path / "good.hm"
+    logm.save(good)
+    raw = bytearray(good.read_bytes())
+    nb = len(logm.blocks)
+    lr = int(np.nonzero(logm.blocks["kind"] != N.BLOCK_DENSE)[0][0])
+    dn = int(np.nonzero(logm.blocks["kind"] == N.BLOCK_DENSE)[0][0])
+    for b, field in ((lr, "off_a"), (lr, "off_b"), (dn, "off_a"), (lr, "row_ofs"), (lr, "col_ofs")):
+        bad = bytearray(raw)
+        o = 72 + 72 * b + 8 * RECORD_FIELDS.index(field)
+        bad[o:o + 8] = struct.pack("<q", (1 << 63) - 5)
+        _rewrite_checksums(bad, nb)
+        path = tmp_path / f"wrap_{field}.hm"
+        path.write_bytes(bytes(bad))
+        assert _read_rc(path, nb, logm.arena.size) == N.HMVM_OK      # the file itself is consistent
+        p = C.c_void_p()
+        opts = N.HmvmPlanOpts()
+        opts.precision, opts.check_finite = N.PREC_FP64, 1
+        rc = N.hmvm().hmvm_plan_create_from_file(str(path).encode(), C.byref(opts), -1, C.byref(p))
+        assert rc == 2, (field, rc)                                   # HMVM_ERR_BAD_BLOCK
