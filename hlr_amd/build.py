@@ -11,6 +11,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -48,14 +49,19 @@ def build_hlrgen() -> Path:
 def build_hmvm() -> Path:
     LIBDIR.mkdir(exist_ok=True)
     dev = CSRC / "device"
-    objs = []
-    for src in ("plan_build.cpp", "hmfile.cpp", "hmvm.hip", "mrhs.hip", "batch.hip", "vecops.hip"):
-        obj = LIBDIR / (Path(src).stem + ".o")
+    srcs = ("plan_build.cpp", "hmfile.cpp", "hmvm.hip", "mrhs.hip", "batch.hip", "vecops.hip")
+    objs = [str(LIBDIR / (Path(src).stem + ".o")) for src in srcs]
+
+    def compile_one(src: str, obj: str) -> None:
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         flags = DEVICE_FLAGS + [f"--offload-arch={ARCH}"] if src.endswith(".hip") else []
         _run([HIPCC, "-O3", "-fPIC", "-fopenmp", "-std=c++17", "-march=x86-64-v3",
-              f"-I{INCLUDE}", f"-I{dev}", *flags, *lang, "-c", str(dev / src), "-o", str(obj)])
-        objs.append(str(obj))
+              f"-I{INCLUDE}", f"-I{dev}", *flags, *lang, "-c", str(dev / src), "-o", obj])
+
+    # the translation units are independent: compile them side by side
+    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1)) as pool:
+        for f in [pool.submit(compile_one, s, o) for s, o in zip(srcs, objs)]:
+            f.result()
     out = LIBDIR / "libhmvm.so"
     tmp = out.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fopenmp", f"--offload-arch={ARCH}", *objs, "-o", str(tmp)])

@@ -591,7 +591,14 @@ int launch_apply(const hmvm_plan* p, double alpha, int op, const XT* x, YT* y, d
 int64_t call_ws_doubles(const hmvm_plan* p, int op, int nrhs) {
     const bool trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
     int64_t w = ws_doubles(p, op);
-    if (!trans && nrhs > 1) w = std::max(w, mrhs_ws_doubles(p->dp, std::min(nrhs, kMrhsMax)));
+    if (!trans && nrhs > 1) {
+        // sizes from the host layout, so that host-only plans answer too
+        DevPlan d{};
+        d.psize = p->L.psize;
+        d.ntent = p->L.ntent;
+        d.ncols = p->L.ncols;
+        w = std::max(w, mrhs_ws_doubles(d, std::min(nrhs, kMrhsMax)));
+    }
     return w;
 }
 
