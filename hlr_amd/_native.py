@@ -18,12 +18,14 @@ ERRORS = {
     1: "HMVM_ERR_INVALID_ARG", 2: "HMVM_ERR_BAD_BLOCK", 3: "HMVM_ERR_OVERLAP",
     4: "HMVM_ERR_NONFINITE", 5: "HMVM_ERR_ALLOC", 6: "HMVM_ERR_DEVICE",
     7: "HMVM_ERR_UNSUPPORTED", 8: "HMVM_ERR_LAYOUT", 9: "HMVM_ERR_KERNEL",
-    10: "HMVM_ERR_IO", 11: "HMVM_ERR_CORRUPT",
+    10: "HMVM_ERR_IO", 11: "HMVM_ERR_CORRUPT", 12: "HMVM_ERR_TOO_LARGE",
 }
 OP_NORMAL, OP_TRANSPOSED, OP_ADJOINT, OP_CONJUGATE = 0, 1, 2, 3
 BLOCK_DENSE, BLOCK_LOWRANK, BLOCK_LOWRANK_SV = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_VALR, PREC_BFP = 0, 1, 2, 3
 F64, F32 = 0, 1
+# size of the fixed device scratch of hmvm_apply_noalloc (HMVM_NOALLOC_MAX_BYTES)
+NOALLOC_MAX_BYTES = 64 * 1024 * 1024
 FMT_F64, FMT_F32, FMT_BF16, FMT_FP8 = 0, 1, 2, 3
 FMT_B8, FMT_B16, FMT_B25 = 4, 5, 6
 NFMT = 7
@@ -98,6 +100,7 @@ HMVM_SYMBOLS = [
     ("hmvm_apply_timed", _i32, [_vp, C.c_double, _i32, _vp, _vp, _i32, _vp, C.POINTER(C.c_float)]),
     ("hmvm_workspace_size", _i32, [_vp, _i32, _i32, C.POINTER(_i64)]),
     ("hmvm_apply_ws", _i32, [_vp, C.c_double, _i32, _vp, _i64, _vp, _i64, _i32, _i32, _vp, _i64, _vp]),
+    ("hmvm_apply_noalloc", _i32, [_vp, C.c_double, _i32, _vp, _i64, _vp, _i64, _i32, _i32, _vp]),
     ("hmvm_tlr_apply", _i32, [_vp, C.c_double, _i32, _vp, _vp, _i32, _vp]),
     ("hmvm_grad", _i32, [_vp, _vp, _vp, _vp, C.POINTER(_vp), _i32, _vp]),
     ("hmvm_plan_arena_elems", _i32, [_vp, _i32, C.POINTER(_i64)]),

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -645,6 +646,37 @@ int apply_impl(const hmvm_plan* p, double alpha, int op, const void* x, int64_t 
     return HMVM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// fixed device scratch of hmvm_apply_noalloc
+// ---------------------------------------------------------------------------
+
+// One array per GPU (each device that loads the library holds its own copy
+// until the process ends).  Host code never passes `scratch` itself to a
+// kernel: on the host the name stands for a host-side shadow, so its device
+// address is looked up once per device with hipGetSymbolAddress and kept in
+// g_scratch_dev.
+static_assert(HMVM_NOALLOC_MAX_BYTES % 256 == 0, "scratch size is a multiple of its alignment");
+__device__ __align__(256) double scratch[HMVM_NOALLOC_MAX_BYTES / sizeof(double)];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};
+
+// device address of the scratch array of device `dev`, which must be the
+// current device
+int scratch_address(int dev, void** out) {
+    *out = nullptr;
+    if (dev < 0 || dev >= kMaxScratchDevices) return HMVM_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* a = nullptr;
+        if (hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)) != hipSuccess || !a) return HMVM_ERR_DEVICE;
+        g_scratch_dev[dev] = a;
+    }
+    *out = g_scratch_dev[dev];
+    return HMVM_OK;
+}
+
 int check_apply_args(const hmvm_plan* p, int op, const void* x, const void* y, int nrhs, int vdt) {
     if (!p) return HMVM_ERR_INVALID_ARG;
     if (op < HMVM_OP_NORMAL || op > HMVM_OP_CONJUGATE) return HMVM_ERR_INVALID_ARG;
@@ -681,6 +713,7 @@ const char* hmvm_error_string(int code) {
     case HMVM_ERR_KERNEL: return "kernel reported a failed check";
     case HMVM_ERR_IO: return "file missing, short, or not an H-matrix file";
     case HMVM_ERR_CORRUPT: return "H-matrix file fails its checksums";
+    case HMVM_ERR_TOO_LARGE: return "workspace of the call exceeds the fixed scratch buffer (HMVM_NOALLOC_MAX_BYTES)";
     default: return "unknown error";
     }
 }
@@ -788,6 +821,22 @@ int hmvm_apply(const hmvm_plan* p, double alpha, int op, const void* x, int64_t 
     rc = apply_impl(p, alpha, op, x, ldx, y, ldy, nrhs, vdt, ws, bytes, st);
     if (hipFreeAsync(ws, st) != hipSuccess && rc == HMVM_OK) rc = HMVM_ERR_DEVICE;
     return rc;
+}
+
+int hmvm_apply_noalloc(const hmvm_plan* p, double alpha, int op, const void* x, int64_t ldx, void* y, int64_t ldy,
+                       int nrhs, int vdt, void* stream) {
+    int rc = check_apply_args(p, op, x, y, nrhs, vdt);
+    if (rc != HMVM_OK) return rc;
+    if (alpha == 0.0) return HMVM_OK;
+    const int64_t bytes = call_ws_doubles(p, op, nrhs) * int64_t(sizeof(double));
+    if (bytes > HMVM_NOALLOC_MAX_BYTES) return HMVM_ERR_TOO_LARGE;
+    DeviceGuard g(p->device);
+    if (!g.ok) return HMVM_ERR_DEVICE;
+    void* ws = nullptr;
+    if ((rc = scratch_address(p->device, &ws)) != HMVM_OK) return rc;
+    // every workspace value the kernels read is written earlier in the same
+    // call, so nothing is cleared between calls
+    return apply_impl(p, alpha, op, x, ldx, y, ldy, nrhs, vdt, ws, bytes, static_cast<hipStream_t>(stream));
 }
 
 int hmvm_apply_timed(const hmvm_plan* p, double alpha, int op, const void* x, void* y, int vdt, void* stream,

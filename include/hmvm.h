@@ -43,7 +43,8 @@ enum {
     HMVM_ERR_LAYOUT         = 8,   /* leading dimension too small / misaligned  */
     HMVM_ERR_KERNEL         = 9,   /* a kernel reported a failed internal check */
     HMVM_ERR_IO             = 10,  /* file missing, short, or not an H-matrix file */
-    HMVM_ERR_CORRUPT        = 11   /* H-matrix file fails its checksums         */
+    HMVM_ERR_CORRUPT        = 11,  /* H-matrix file fails its checksums         */
+    HMVM_ERR_TOO_LARGE      = 12   /* workspace beyond HMVM_NOALLOC_MAX_BYTES   */
 };
 
 /* matrix operation, values as Hpro::matop_t */
@@ -206,6 +207,27 @@ int hmvm_apply_ws(const hmvm_plan* plan, double alpha, int op,
                   const void* x, int64_t ldx, void* y, int64_t ldy,
                   int nrhs, int vec_dtype, void* workspace, int64_t ws_bytes,
                   void* stream);
+
+/* size of the fixed device scratch buffer of hmvm_apply_noalloc (per GPU) */
+#define HMVM_NOALLOC_MAX_BYTES (64LL * 1024 * 1024)
+
+/*
+ * hmvm_apply that allocates nothing: no hipMalloc, hipFree or allocating
+ * library call.  Its workspace is a fixed device array of
+ * HMVM_NOALLOC_MAX_BYTES that the library holds on every GPU it runs on
+ * until the process ends.  Arguments and results are those of hmvm_apply,
+ * with the same kernels, so the results are bitwise those of hmvm_apply.
+ * A call whose workspace (hmvm_workspace_size) exceeds
+ * HMVM_NOALLOC_MAX_BYTES fails with HMVM_ERR_TOO_LARGE before any device
+ * work.  All calls on one GPU share that GPU's array: calls that overlap in
+ * time on one GPU (different streams or host threads) race and give wrong
+ * results; serialise them.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).
+ */
+int hmvm_apply_noalloc(const hmvm_plan* plan, double alpha, int op,
+                       const void* x, int64_t ldx, void* y, int64_t ldy,
+                       int nrhs, int vec_dtype, void* stream);
 
 /*
  * Tile-low-rank product.  Replaces hlr::tlr::mul_vec (arith/mulvec.hh:882-919,
