@@ -107,6 +107,7 @@ HMVM_SYMBOLS = [
     ("hmvm_plan_get_arena", _i32, [_vp, _i32, _vp, _vp]),
     ("hmvm_plan_set_arena", _i32, [_vp, _i32, _vp, _vp]),
     ("hmvm_plan_unpack_block", _i32, [_vp, _i64, C.POINTER(_vp), _i32, _vp, _vp]),
+    ("hmvm_plan_arena_source", _i32, [_vp, _i32, _vp]),
     ("hmvm_batch_create", _i32, [C.POINTER(_vp), _i32, C.POINTER(_i32), _i32, C.POINTER(_vp)]),
     ("hmvm_batch_destroy", _i32, [_vp]),
     ("hmvm_batch_apply", _i32, [_vp, C.POINTER(C.c_double), C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_vp),

@@ -980,6 +980,15 @@ int hmvm_plan_unpack_block(const hmvm_plan* p, int64_t block, const void* const 
     return unpack_block(p->L, block, host_arenas, as_f64 != 0, a, b);
 }
 
+int hmvm_plan_arena_source(const hmvm_plan* p, int fmt, int64_t* src) {
+    if (!p) return HMVM_ERR_INVALID_ARG;
+    try {
+        return arena_source(p->L, fmt, src);
+    } catch (const std::bad_alloc&) {
+        return HMVM_ERR_ALLOC;
+    }
+}
+
 } // extern "C"
 
 namespace hmvm {

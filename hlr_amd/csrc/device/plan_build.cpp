@@ -906,3 +906,90 @@ int unpack_block(const HostLayout& L, int64_t bi, const void* const arenas[NFMT]
 }
 
 } // namespace hmvm
+
+namespace hmvm {
+
+// For every stored value of format f: its offset in the FP64 arena that
+// hmvm_plan_create was given (the leaves' own column-major layout), or -1
+// for padding.  One walk over the R chunks and V segments.
+int arena_source(const HostLayout& L, int f, int64_t* src) {
+    if (f < 0 || f >= NFMT || (L.nvals[f] > 0 && !src)) return HMVM_ERR_INVALID_ARG;
+    const int64_t nv = L.nvals[f];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nv; ++k) src[k] = -1;
+    if (nv == 0) return HMVM_OK;
+    const int64_t nb = int64_t(L.blocks.size());
+    // t-entry -> block
+    std::vector<int64_t> ent_block(static_cast<size_t>(L.ntent), -1);
+    for (int64_t b = 0; b < nb; ++b)
+        if (L.blk_t0[b] >= 0)
+            for (int64_t l = 0; l < L.blocks[b].rank; ++l) ent_block[size_t(L.blk_t0[b] + l)] = b;
+    // dense blocks of each row panel, by first column
+    const int64_t NR = int64_t(L.rp.size());
+    std::vector<int64_t> prow0(static_cast<size_t>(NR));
+    for (int64_t p = 0; p < NR; ++p) prow0[size_t(p)] = L.rp[p].row0;
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> pdense(static_cast<size_t>(NR));
+    for (int64_t b = 0; b < nb; ++b) {
+        const hmvm_block& B = L.blocks[b];
+        if (B.kind != HMVM_BLOCK_DENSE) continue;
+        auto it = std::lower_bound(prow0.begin(), prow0.end(), B.row_ofs);
+        for (; it != prow0.end() && *it < B.row_ofs + B.nrows; ++it) pdense[size_t(it - prow0.begin())].push_back({B.col_ofs, b});
+    }
+    for (auto& v : pdense) std::sort(v.begin(), v.end());
+    const int W = fmt_width(f);
+    // R side (row panels write disjoint runs)
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : bad)
+    for (int64_t p = 0; p < NR; ++p) {
+        const RowPanel& P = L.rp[p];
+        const auto& dl = pdense[size_t(p)];
+        for (int64_t ci = P.chunk0; ci < P.chunk0 + P.nchunks; ++ci) {
+            const RChunk& C = L.rchunks[ci];
+            if (C.fmt != f) continue;
+            for (int64_t c = 0; c < int64_t(C.ngroups) * W; ++c) {
+                const int32_t s = L.colsrc[C.col0 + c];
+                if (s == kPadCol) continue;
+                int64_t b = -1, col = -1;
+                if (s >= 0) {
+                    auto it = std::upper_bound(dl.begin(), dl.end(), std::make_pair(int64_t(s), INT64_MAX));
+                    if (it == dl.begin()) { bad = 1; continue; }
+                    --it;
+                    b = it->second;
+                    col = s - L.blocks[b].col_ofs;
+                    if (col >= L.blocks[b].ncols) { bad = 1; continue; }
+                } else {
+                    const int64_t e = ~int64_t(s);
+                    b = ent_block[size_t(e)];
+                    col = e - L.blk_t0[b];
+                }
+                const hmvm_block& B = L.blocks[b];
+                const int64_t g = c / W, w = c % W;
+                for (int64_t i = 0; i < P.m; ++i)
+                    src[C.data_off + g * P.m * W + i * W + w] = B.off_a + col * B.nrows + (P.row0 - B.row_ofs) + i;
+            }
+        }
+    }
+    if (bad) return HMVM_ERR_KERNEL;
+    // V side (column panels write disjoint segments)
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < int64_t(L.cp.size()); ++q) {
+        const ColPanel& C = L.cp[q];
+        for (int64_t j = C.item0; j < C.item0 + C.nitems; ++j) {
+            const VItem& v = L.vitems[j];
+            if (v.fmt != f) continue;
+            const int64_t ld = int64_t(v.ng) * W;
+            for (int64_t cc = 0; cc < ld; ++cc) {
+                const int32_t e = L.vsrc[C.vcol0[f] + int64_t(v.g0) * W + cc];
+                if (e < 0) continue;
+                const int64_t b = ent_block[size_t(e)];
+                const hmvm_block& B = L.blocks[b];
+                const int64_t l = e - L.blk_t0[b];
+                for (int64_t i = 0; i < C.m; ++i)
+                    src[v.data_off + i * ld + cc] = B.off_b + l * B.ncols + (C.col0 - B.col_ofs) + i;
+            }
+        }
+    }
+    return HMVM_OK;
+}
+
+} // namespace hmvm

@@ -103,6 +103,9 @@ int build_layout(const hmvm_block* blocks, int64_t nblocks, int64_t nrows, int64
 // dequantise one block's factors from (host copies of) the arenas
 int unpack_block(const HostLayout& L, int64_t b, const void* const arenas[NFMT], bool as_f64, double* a, double* bb);
 
+// for every stored value of format f, its offset in the plan's input arena (-1: padding)
+int arena_source(const HostLayout& L, int f, int64_t* src);
+
 // storage helpers shared with tests
 uint8_t encode_e4m3(float v);
 float decode_e4m3(uint8_t c);

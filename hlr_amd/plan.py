@@ -264,6 +264,16 @@ class Plan:
             return a.reshape(n, m).T, None
         return a.reshape(k, m).T, bb.reshape(k, n).T
 
+    def arena_source(self, fmt: int) -> np.ndarray:
+        """int64 array, one entry per stored value of format fmt: the offset
+        of that value in the FP64 arena the plan was built from (the leaves'
+        column-major factors), or -1 for padding."""
+        n = self.arena_elems(fmt)
+        src = np.empty(n, dtype=np.int64)
+        N.check(N.hmvm().hmvm_plan_arena_source(self._p, fmt, C.c_void_p(src.ctypes.data) if n else None),
+                "hmvm_plan_arena_source")
+        return src
+
     def grad(self, x, ybar, gx=None, factors: bool = True, stream=None):
         """Adjoint of y = A x: gx += A^T ybar, and per-format FP64 gradient
         arenas in the plan's layout.  For FP64, FP32 and BF16 storage an

@@ -278,6 +278,18 @@ int hmvm_plan_unpack_block(const hmvm_plan* plan, int64_t block,
                            const void* const host_arenas[HMVM_NFMT], int as_f64,
                            double* a, double* b);
 
+/*
+ * The plan's layout as a map: for each of the hmvm_plan_arena_elems(fmt)
+ * stored values of format fmt, src[k] is the offset, in the FP64 arena that
+ * hmvm_plan_create was given, of the value it stores (the leaves' own
+ * column-major factors at off_a / off_b), or -1 for padding.  Host-side;
+ * works on host-only plans.  With it a caller moves whole arenas between
+ * the leaves' layout and the plan's (gradients from hmvm_grad, factor
+ * directions for set_arena) without per-block calls.  No counterpart in the
+ * reference (its leaves keep their factors; matrix/lrmatrix.hh:175-204).
+ */
+int hmvm_plan_arena_source(const hmvm_plan* plan, int fmt, int64_t* src);
+
 /* ------------------------------------------------------------------------ */
 /* batching: many plans, many right-hand sides, one launch sequence         */
 /* ------------------------------------------------------------------------ */
