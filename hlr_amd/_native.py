@@ -76,6 +76,31 @@ def block_dtype():
                      ("off_s", "<i8")], align=True)
 
 
+# numpy dtypes matching hmvm_cluster and hmvm_ublock
+BLOCK_COUPLING = 3
+
+
+def cluster_dtype():
+    import numpy as np
+    return np.dtype([("ofs", "<i8"), ("size", "<i4"), ("rank", "<i4"), ("parent", "<i4"), ("pad_", "<i4"),
+                     ("off_basis", "<i8"), ("off_transfer", "<i8"), ("off_s", "<i8")], align=True)
+
+
+def ublock_dtype():
+    import numpy as np
+    return np.dtype([("row_ofs", "<i8"), ("col_ofs", "<i8"), ("nrows", "<i4"), ("ncols", "<i4"),
+                     ("kind", "<i4"), ("rowcb", "<i4"), ("colcb", "<i4"), ("pad_", "<i4"), ("off", "<i8")],
+                    align=True)
+
+
+class HmvmUplanInfo(C.Structure):
+    _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("nrow_clusters", C.c_int64),
+                ("ncol_clusters", C.c_int64), ("ncoupling", C.c_int64), ("ndense", C.c_int64),
+                ("nested", C.c_int32), ("levels", C.c_int32), ("basis_bytes", C.c_int64),
+                ("transfer_bytes", C.c_int64), ("coupling_bytes", C.c_int64), ("dense_bytes", C.c_int64),
+                ("flops", C.c_double), ("datasize", C.c_double)]
+
+
 class HmvmError(RuntimeError):
     def __init__(self, code: int, what: str = ""):
         self.code = code
@@ -127,6 +152,11 @@ HMVM_SYMBOLS = [
                                       C.POINTER(_i64)]),
     ("hmvm_hmatrix_read", _i32, [C.c_char_p, _vp, _i64, _vp, _i64]),
     ("hmvm_plan_create_from_file", _i32, [C.c_char_p, _vp, _i32, C.POINTER(_vp)]),
+    ("hmvm_uplan_create", _i32, [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _vp, _i64, C.POINTER(HmvmPlanOpts),
+                                  _i32, C.POINTER(_vp)]),
+    ("hmvm_uplan_destroy", _i32, [_vp]),
+    ("hmvm_uplan_get_info", _i32, [_vp, C.POINTER(HmvmUplanInfo)]),
+    ("hmvm_uplan_apply", _i32, [_vp, C.c_double, _i32, _vp, _vp, _vp]),
     ("hmvm_error_string", C.c_char_p, [_i32]),
 ]
 

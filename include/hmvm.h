@@ -429,6 +429,93 @@ int hmvm_hmatrix_read(const char* path, hmvm_block* blocks, int64_t nblocks,
 int hmvm_plan_create_from_file(const char* path, const hmvm_plan_opts* opts,
                                int device, hmvm_plan** out);
 
+/* ------------------------------------------------------------------------ */
+/* uniform-H and H^2 matrices: shared or nested cluster bases               */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * One cluster of a row or column cluster-basis tree
+ * (matrix/shared_cluster_basis.hh, matrix/nested_cluster_basis.hh).
+ * Shared bases (uniform-H): every cluster with rank > 0 has an explicit
+ * basis and parent == -1.  Nested bases (H^2): leaves have an explicit
+ * basis; a cluster with a parent has the transfer matrix E (rank x parent
+ * rank, column-major) with which its parent's basis restricted to its
+ * indices is (own basis) * E (cb.transfer_mat(i) of the parent, son i).
+ */
+typedef struct hmvm_cluster {
+    int64_t ofs;           /* first index of the cluster                        */
+    int32_t size;          /* number of indices                                 */
+    int32_t rank;          /* basis columns k (0: no basis)                     */
+    int32_t parent;        /* nested bases: index of the parent, else -1        */
+    int32_t pad_;
+    int64_t off_basis;     /* explicit basis, size x rank column-major, or -1   */
+    int64_t off_transfer;  /* nested bases: E, rank x parent rank, or -1        */
+    int64_t off_s;         /* singular values of the basis (rank), or -1        */
+} hmvm_cluster;
+
+/* leaf of a uniform-H / H^2 matrix: a dense block, or a coupling matrix S
+   (row basis rank x column basis rank, column-major) between row cluster
+   `rowcb` and column cluster `colcb`, whose index sets are the block's
+   (matrix/uniform_lrmatrix.hh, matrix/h2_lrmatrix.hh: M = U S V^H) */
+typedef struct hmvm_ublock {
+    int64_t row_ofs;
+    int64_t col_ofs;
+    int32_t nrows;
+    int32_t ncols;
+    int32_t kind;          /* HMVM_BLOCK_DENSE or HMVM_BLOCK_COUPLING           */
+    int32_t rowcb;         /* coupling: index into the row clusters             */
+    int32_t colcb;         /* coupling: index into the column clusters          */
+    int32_t pad_;
+    int64_t off;           /* dense: D (m x n); coupling: S; column-major       */
+} hmvm_ublock;
+
+#define HMVM_BLOCK_COUPLING 3
+
+typedef struct hmvm_uplan hmvm_uplan;
+
+/*
+ * Device plan of a uniform-H (shared bases) or H^2 (nested bases) matrix.
+ * The product runs as stream-ordered device passes: forward transform of
+ * x (explicit bases, then transfers level by level up the column tree),
+ * coupling products, backward transform (transfers down the row tree, then
+ * explicit bases) and the dense near field.  opts as in hmvm_plan_create;
+ * under HMVM_PREC_VALR the bases' singular values (off_s) choose per-column
+ * formats.  device -1 builds and checks a host-only plan (no apply).
+ */
+int hmvm_uplan_create(const hmvm_cluster* rowcb, int64_t nrowcb,
+                      const hmvm_cluster* colcb, int64_t ncolcb,
+                      const hmvm_ublock* blocks, int64_t nblocks,
+                      int64_t nrows, int64_t ncols,
+                      const double* arena, int64_t arena_len,
+                      const hmvm_plan_opts* opts, int device, hmvm_uplan** out);
+int hmvm_uplan_destroy(hmvm_uplan* plan);
+
+/* statistics: factor bytes of bases, transfers, couplings and dense
+   leaves, mul_vec_flops and mul_vec_datasize (arith/uniform.hh,
+   arith/h2.hh:140-178) */
+typedef struct hmvm_uplan_info {
+    int64_t nrows, ncols;
+    int64_t nrow_clusters, ncol_clusters, ncoupling, ndense;
+    int32_t nested;              /* 1: H^2 (transfer matrices), 0: uniform-H      */
+    int32_t levels;              /* transfer levels (row tree + column tree)      */
+    int64_t basis_bytes;         /* explicit bases, as stored                     */
+    int64_t transfer_bytes;
+    int64_t coupling_bytes;
+    int64_t dense_bytes;
+    double  flops;
+    double  datasize;
+} hmvm_uplan_info;
+int hmvm_uplan_get_info(const hmvm_uplan* plan, hmvm_uplan_info* info);
+
+/*
+ * y += alpha * op(M) * x for FP64 device vectors (one right-hand side).
+ * Replaces hlr::uniform::mul_vec (arith/uniform.hh:114-145) and
+ * hlr::h2::mul_vec (arith/h2.hh:18-44): alpha == 0 returns at once,
+ * op as for hmvm_apply.  Workspace is a stream-ordered allocation.
+ */
+int hmvm_uplan_apply(const hmvm_uplan* plan, double alpha, int op,
+                     const double* x, double* y, void* stream);
+
 const char* hmvm_error_string(int code);
 
 #ifdef __cplusplus

@@ -15,6 +15,10 @@ header and exercised by tests/test_oracle.py):
                                            the OCP E4M3 code table)
   bfp_exp, bfp_round, bfp_format           pinned (numpy rint, brute-force
                                            minimality, representable columns)
+  uniform_mul_vec, uniform_dense_expand    pinned (numpy products of a
+                                           hand-assembled expansion; H^2 equal
+                                           to the uniform-H matrix with the
+                                           expanded bases)
 """
 from __future__ import annotations
 
@@ -69,6 +73,10 @@ def lib() -> C.CDLL:
         L.oracle_bfp_round.restype = d
         L.oracle_bfp_format.argtypes = [i32, vp, vp, vp, d, i32]
         L.oracle_bfp_format.restype = i32
+        L.oracle_uniform_mul_vec.argtypes = [vp, i64, vp, i64, vp, i64, vp, d, i32, vp, vp]
+        L.oracle_uniform_mul_vec.restype = i32
+        L.oracle_uniform_dense_expand.argtypes = [vp, i64, vp, i64, vp, i64, vp, i64, i64, vp]
+        L.oracle_uniform_dense_expand.restype = i32
         _LIB = L
     return _LIB
 
@@ -199,3 +207,22 @@ def bfp_format(cols, tol, rate, piece=64):
     npc = np.ascontiguousarray(npieces, dtype=np.int64)
     pl = np.ascontiguousarray(plen, dtype=np.int64)
     return lib().oracle_bfp_format(len(cols), _p(v), _p(npc), _p(pl), float(tol), int(rate))
+
+
+def uniform_mul_vec(rowcb, colcb, blocks, arena, alpha, op, x, y):
+    """y += alpha op(M) x for a uniform-H / H^2 matrix (hmvm_cluster and
+    hmvm_ublock record arrays), y modified in place."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    assert y.dtype == np.float64 and y.flags.c_contiguous
+    rc = lib().oracle_uniform_mul_vec(_p(rowcb), len(rowcb), _p(colcb), len(colcb), _p(blocks), len(blocks),
+                                      _p(arena), float(alpha), int(op), _p(x), _p(y))
+    assert rc == 0, rc
+    return y
+
+
+def uniform_dense_expand(rowcb, colcb, blocks, arena, nrows, ncols):
+    A = np.zeros((ncols, nrows))
+    rc = lib().oracle_uniform_dense_expand(_p(rowcb), len(rowcb), _p(colcb), len(colcb), _p(blocks), len(blocks),
+                                           _p(arena), nrows, ncols, _p(A))
+    assert rc == 0, rc
+    return A.T

@@ -15,11 +15,14 @@ HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
 
 
+# compiler flags of the restatement (quoted by bench.py's cpu_baseline)
+CFLAGS = ["-O3", "-fPIC", "-fopenmp", "-march=x86-64-v3", "-std=c11"]
+
+
 def build() -> Path:
     out = HERE / "liboracle.so"
     tmp = out.with_suffix(".so.tmp")
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-march=x86-64-v3", "-std=c11",
-           f"-I{ROOT / 'include'}", str(HERE / "hmvm_oracle.c"), "-o", str(tmp), "-lm"]
+    cmd = ["gcc", *CFLAGS, "-shared", f"-I{ROOT / 'include'}", str(HERE / "hmvm_oracle.c"), "-o", str(tmp), "-lm"]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(" ".join(cmd) + "\n" + proc.stdout + proc.stderr)

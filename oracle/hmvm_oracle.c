@@ -35,13 +35,24 @@
  *                         given relative 2-norm tolerance (the VALR
  *                         tolerance u_l of a low-rank column)
  *   oracle_dense_expand   sum of the leaves placed at their offsets
+ *   oracle_uniform_mul_vec  hlr::uniform::mul_vec (arith/uniform.hh:114-145;
+ *                         arith/detail/uniform.hh:53-193) and hlr::h2::mul_vec
+ *                         (arith/h2.hh:18-44; arith/detail/h2.hh:29-322):
+ *                         forward transform, couplings and dense leaves in
+ *                         block order, backward transform
+ *   oracle_uniform_dense_expand  dense leaves plus U_r S V_c^H per coupling,
+ *                         nested bases expanded through their transfers
  *
  * Pins (tests/): dense expansion against direct kernel evaluation and
  * numpy products; linearity, accumulate, alpha = 0, <Ax,w> = <x,A^T w>,
  * TLR == H on one-level trees, brute force on hand-built trees; rounding
  * against torch's float8_e4m3fn / bfloat16 conversions and the OCP table;
  * block floating point against numpy's rint, brute-force minimality of the
- * step and of the chosen width, and exactly representable columns.
+ * step and of the chosen width, and exactly representable columns;
+ * uniform-H / H^2 products against numpy products of the dense expansion
+ * and of a hand-assembled expansion (bases, transfers and couplings
+ * multiplied out in numpy), and H^2 against the equal uniform-H matrix
+ * whose bases are the expanded nested ones.
  */
 #include <math.h>
 #include <omp.h>
@@ -491,4 +502,173 @@ int oracle_bfp_format(int ncols, const double* vals, const int64_t* npieces, con
         if (ok) return HMVM_FMT_B8 + f;
     }
     return rate <= 23 ? HMVM_FMT_F32 : HMVM_FMT_F64;
+}
+
+/* ------------------------------------------------------------------------ */
+/* uniform-H (shared bases) and H^2 (nested bases)                          */
+/* ------------------------------------------------------------------------ */
+
+/* depth of every cluster in its basis tree (-1 on a cycle or a bad parent) */
+static int cluster_depths(const hmvm_cluster* cb, int64_t n, int* depth) {
+    for (int64_t i = 0; i < n; ++i) {
+        int d = 0;
+        int64_t p = cb[i].parent;
+        while (p >= 0) {
+            if (p >= n || ++d > n) return -1;
+            p = cb[p].parent;
+        }
+        depth[i] = d;
+    }
+    return 0;
+}
+
+/*
+ * oracle_uniform_mul_vec: y += alpha op(M) x for a uniform-H or H^2 matrix.
+ *   forward transform (scalar_to_uniform, arith/detail/uniform.hh:121-146
+ *     for shared bases, arith/detail/h2.hh:238-286 for nested ones): the
+ *     coefficients of every cluster of the input tree, xh_c = B_c^H x|c for
+ *     an explicit basis, xh_c = sum over sons s of E_s^H xh_s otherwise
+ *     (sons before parents);
+ *   leaves in block order (arith/detail/uniform.hh:53-118,
+ *     arith/detail/h2.hh:29-89): a coupling adds alpha op(S) xh to the
+ *     output coefficients (uniform_lrmatrix::uni_apply_add), a dense leaf
+ *     adds alpha op(D) x|in to y|out directly;
+ *   backward transform (add_uniform_to_scalar, arith/detail/uniform.hh:
+ *     171-193, arith/detail/h2.hh:289-322): parents before sons, the
+ *     accumulated coefficient s_c = yh_c + E_c s_parent, and y|c += B_c s_c
+ *     for an explicit basis.
+ * The input tree is the column tree for op normal, the row tree for
+ * transposed / adjoint (real values: conjugate == normal).
+ */
+int oracle_uniform_mul_vec(const hmvm_cluster* rowcb, int64_t nr, const hmvm_cluster* colcb, int64_t nc,
+                           const hmvm_ublock* blocks, int64_t nb, const double* arena, double alpha, int op,
+                           const double* x, double* y) {
+    if (alpha == 0.0) return 0;
+    const int trans = (op == HMVM_OP_TRANSPOSED || op == HMVM_OP_ADJOINT);
+    const hmvm_cluster* in = trans ? rowcb : colcb;
+    const hmvm_cluster* out = trans ? colcb : rowcb;
+    const int64_t nin = trans ? nr : nc, nout = trans ? nc : nr;
+    int* din = (int*)calloc((size_t)(nin + 1), sizeof(int));
+    int* dout = (int*)calloc((size_t)(nout + 1), sizeof(int));
+    int64_t* ein = (int64_t*)calloc((size_t)(nin + 1), sizeof(int64_t));
+    int64_t* eout = (int64_t*)calloc((size_t)(nout + 1), sizeof(int64_t));
+    int rc = 0;
+    double *xh = NULL, *yh = NULL;
+    if (!din || !dout || !ein || !eout || cluster_depths(in, nin, din) || cluster_depths(out, nout, dout)) {
+        rc = 1;
+        goto done;
+    }
+    for (int64_t i = 0; i < nin; ++i) ein[i + 1] = ein[i] + in[i].rank;
+    for (int64_t i = 0; i < nout; ++i) eout[i + 1] = eout[i] + out[i].rank;
+    xh = (double*)calloc((size_t)(ein[nin] + 1), sizeof(double));
+    yh = (double*)calloc((size_t)(eout[nout] + 1), sizeof(double));
+    if (!xh || !yh) { rc = 1; goto done; }
+    int maxd = 0;
+    for (int64_t i = 0; i < nin; ++i) maxd = din[i] > maxd ? din[i] : maxd;
+    for (int64_t i = 0; i < nout; ++i) maxd = dout[i] > maxd ? dout[i] : maxd;
+    /* forward transform, deepest clusters first */
+    for (int d = maxd; d >= 0; --d)
+        for (int64_t i = 0; i < nin; ++i) {
+            const hmvm_cluster* c = &in[i];
+            if (din[i] != d || c->rank == 0) continue;
+            if (c->off_basis >= 0)
+                gemv_t(c->size, c->rank, 1.0, arena + c->off_basis, c->size, x + c->ofs, xh + ein[i]);
+            if (c->parent >= 0 && c->off_transfer >= 0 && in[c->parent].rank > 0)
+                /* xh_parent += E^H xh_c, E rank x parent rank */
+                gemv_t(c->rank, in[c->parent].rank, 1.0, arena + c->off_transfer, c->rank, xh + ein[i],
+                       xh + ein[c->parent]);
+        }
+    /* leaves in block order */
+    for (int64_t b = 0; b < nb; ++b) {
+        const hmvm_ublock* B = &blocks[b];
+        if (B->kind == HMVM_BLOCK_DENSE) {
+            if (!trans) gemv_n(B->nrows, B->ncols, alpha, arena + B->off, B->nrows, x + B->col_ofs, y + B->row_ofs);
+            else gemv_t(B->nrows, B->ncols, alpha, arena + B->off, B->nrows, x + B->row_ofs, y + B->col_ofs);
+        } else {
+            const int64_t kr = rowcb[B->rowcb].rank, kc = colcb[B->colcb].rank;
+            if (kr == 0 || kc == 0) continue;
+            if (!trans)
+                gemv_n(kr, kc, alpha, arena + B->off, kr, xh + ein[B->colcb], yh + eout[B->rowcb]);
+            else
+                gemv_t(kr, kc, alpha, arena + B->off, kr, xh + ein[B->rowcb], yh + eout[B->colcb]);
+        }
+    }
+    /* backward transform, parents first: s_c = yh_c + E_c s_parent */
+    for (int d = 0; d <= maxd; ++d)
+        for (int64_t i = 0; i < nout; ++i) {
+            const hmvm_cluster* c = &out[i];
+            if (dout[i] != d || c->rank == 0) continue;
+            if (c->parent >= 0 && c->off_transfer >= 0 && out[c->parent].rank > 0)
+                gemv_n(c->rank, out[c->parent].rank, 1.0, arena + c->off_transfer, c->rank, yh + eout[c->parent],
+                       yh + eout[i]);
+            if (c->off_basis >= 0)
+                gemv_n(c->size, c->rank, 1.0, arena + c->off_basis, c->size, yh + eout[i], y + c->ofs);
+        }
+done:
+    free(din); free(dout); free(ein); free(eout); free(xh); free(yh);
+    return rc;
+}
+
+/* dense expansion of a uniform-H / H^2 matrix into A (nrows x ncols,
+   column-major): dense leaves, and B_r S B_c^H per coupling with the
+   explicit bases of nested clusters expanded through the transfers */
+static void expand_basis(const hmvm_cluster* cb, int64_t ncb, int64_t i, const double* arena, double* out,
+                         int64_t ld) {
+    /* out (cluster size x rank, leading dimension ld) = explicit basis, or
+       sum over sons s: rows of s get (basis of s) E_s */
+    const hmvm_cluster* c = &cb[i];
+    if (c->off_basis >= 0) {
+        for (int64_t l = 0; l < c->rank; ++l)
+            for (int64_t r = 0; r < c->size; ++r) out[l * ld + r] = arena[c->off_basis + l * c->size + r];
+        return;
+    }
+    for (int64_t l = 0; l < c->rank; ++l)
+        for (int64_t r = 0; r < c->size; ++r) out[l * ld + r] = 0.0;
+    for (int64_t s = 0; s < ncb; ++s) {
+        if (cb[s].parent != i || cb[s].rank == 0 || cb[s].off_transfer < 0) continue;
+        const hmvm_cluster* so = &cb[s];
+        double* bs = (double*)calloc((size_t)so->size * (size_t)so->rank + 1, sizeof(double));
+        expand_basis(cb, ncb, s, arena, bs, so->size);
+        /* out[rows of s, :] += bs (size_s x rank_s) * E_s (rank_s x rank_c) */
+        for (int64_t l = 0; l < c->rank; ++l)
+            for (int64_t j = 0; j < so->rank; ++j) {
+                const double e = arena[so->off_transfer + l * so->rank + j];
+                for (int64_t r = 0; r < so->size; ++r) out[l * ld + (so->ofs - c->ofs) + r] += bs[j * so->size + r] * e;
+            }
+        free(bs);
+    }
+}
+
+int oracle_uniform_dense_expand(const hmvm_cluster* rowcb, int64_t nr, const hmvm_cluster* colcb, int64_t nc,
+                                const hmvm_ublock* blocks, int64_t nb, const double* arena, int64_t nrows,
+                                int64_t ncols, double* A) {
+    memset(A, 0, sizeof(double) * (size_t)(nrows * ncols));
+    for (int64_t b = 0; b < nb; ++b) {
+        const hmvm_ublock* B = &blocks[b];
+        if (B->kind == HMVM_BLOCK_DENSE) {
+            for (int64_t j = 0; j < B->ncols; ++j)
+                for (int64_t i = 0; i < B->nrows; ++i)
+                    A[(B->col_ofs + j) * nrows + B->row_ofs + i] += arena[B->off + j * B->nrows + i];
+            continue;
+        }
+        const hmvm_cluster* r = &rowcb[B->rowcb];
+        const hmvm_cluster* c = &colcb[B->colcb];
+        if (r->rank == 0 || c->rank == 0) continue;
+        double* U = (double*)calloc((size_t)r->size * (size_t)r->rank, sizeof(double));
+        double* V = (double*)calloc((size_t)c->size * (size_t)c->rank, sizeof(double));
+        if (!U || !V) { free(U); free(V); return 1; }
+        expand_basis(rowcb, nr, B->rowcb, arena, U, r->size);
+        expand_basis(colcb, nc, B->colcb, arena, V, c->size);
+        const double* S = arena + B->off;
+        for (int64_t j = 0; j < c->size; ++j)
+            for (int64_t i = 0; i < r->size; ++i) {
+                double v = 0.0;
+                for (int64_t p = 0; p < r->rank; ++p)
+                    for (int64_t q = 0; q < c->rank; ++q) v += U[p * r->size + i] * S[q * r->rank + p] * V[q * c->size + j];
+                A[(c->ofs + j) * nrows + r->ofs + i] += v;
+            }
+        free(U);
+        free(V);
+    }
+    return 0;
 }
