@@ -60,8 +60,10 @@ def main() -> int:
     ap.add_argument("--mrhs", type=int, default=32,
                     help="also time each plan with this many right-hand sides per call (0: skip)")
     ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
-    ap.add_argument("--workload", choices=["sphere", "batch", "grad"], default="sphere",
-                    help="sphere: 1/r, n = 262144 (default); batch: 64 Matern matrices, many right-hand sides")
+    ap.add_argument("--workload", choices=["sphere", "matern", "batch", "grad", "uniform"], default="sphere",
+                    help="sphere: 1/r, n = 262144 (default); matern: the plain FP64 drop-in on n = 65536; "
+                         "batch: 64 Matern matrices, many right-hand sides; grad: loss and gradients (TLR); "
+                         "uniform: uniform-H and H^2 forms of the sphere matrix at n = 65536")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -70,6 +72,10 @@ def main() -> int:
         return 2
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "matern":
+        return bench_matern(args)
+    if args.workload == "uniform":
+        return bench_uniform(args)
     if args.workload == "grad":
         return bench_grad(args)
 
@@ -229,18 +235,7 @@ def main() -> int:
     err_gpu = float(max(np.linalg.norm(d[j]) / np.linalg.norm(y64.reshape(R, -1)[j]) for j in range(R)))
     cpu = None
     if not args.no_cpu and R == 1 and OP == N.OP_NORMAL:
-        # cpu_baseline leg: the oracle's restatement of tbb::mul_vec_chunk, timed
-        import oracle
-        cores = min(len(os.sched_getaffinity(0)), 16)
-        yc = np.zeros(hm.n)
-        t1 = oracle.time_chunk(hm.blocks, hm.arena, x_h, yc, cores, 1)
-        nmvm = max(1, int(args.cpu_seconds / max(t1, 1e-6)))
-        tc = oracle.time_chunk(hm.blocks, hm.arena, x_h, yc, cores, nmvm)
-        cpu = {"value": nmvm / tc, "unit": "H-MVM/s", "cores": cores, "kind": "port",
-               "sample": f"{nmvm} FP64 H-MVMs of the same matrix with the oracle's restatement of "
-                         f"tbb::mul_vec_chunk (64-entry locked chunks), {cores} threads, {tc:.1f} s"}
-        log(f"cpu baseline: {cpu['value']:.3f} MVM/s on {cores} threads")
-
+        cpu = cpu_baseline(hm.blocks, hm.arena, x_h, hm.n, args.cpu_seconds)
     # ---- roofline of the dominant kernel: the row-panel pass (k_ufwd) ----
     n = hm.n
     mvm_bytes = imp["datasize"] + 8.0 * 2 * (R - 1) * (n + n)    # vectors counted per right-hand side
@@ -304,6 +299,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": rkernel,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": (f"profiles/{tfile.name} (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE of the "
+                                        "same kernel and workload, copy-calibrated; not measured in this run)")
+                     if traffic is not None else None,
                      "algorithmic_bytes": r_bytes},
         "cpu_baseline": cpu,
     }
@@ -335,6 +333,244 @@ def main() -> int:
             + (f", bfp {mbf[0]:.3f} ms" if "bfp_ms" in mr else ""))
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_matern(args) -> int:
+    """Plain FP64 drop-in: Matern nu = 1.5, l = 0.1, unit variance on
+    n = 65536 points in the unit cube, leaf 64, eta 2, ACA eps 1e-6, FP64
+    factors and accumulation, nrhs 1, x ~ U[-1, 1].  One step is one
+    y += 2 A x with the factors resident in HBM; the relative error is
+    taken against the oracle's FP64 mul_vec on the same matrix, and the
+    restatement of tbb::mul_vec_chunk is timed on the host cores beside it."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from hlr_amd import _native as N
+    from hlr_amd.hmatrix import MATERN_CUBE, HMatrix, random_vector
+    from hlr_amd.plan import Plan
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    hm = HMatrix(MATERN_CUBE)
+    st = hm.stats()
+    log(f"matrix n={hm.n} leaves={st['leaves']} dense={st['dense']} lowrank={st['lowrank']} kmax={st['kmax']} "
+        f"built in {time.time() - t0:.1f} s")
+    x_h = random_vector(hm.n, 2)
+    x = torch.as_tensor(x_h, device=dev)
+    plan = Plan.from_hmatrix(hm, precision="fp64")
+    info = plan.info()
+    ws = torch.empty(plan.workspace_bytes(N.OP_NORMAL, 1) // 8 + 1, dtype=torch.float64, device=dev)
+    y = torch.zeros_like(x)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        plan.apply_ws(2.0, N.OP_NORMAL, x, y, ws)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.apply_ws(2.0, N.OP_NORMAL, x, y, ws)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    phases = np.array([plan.apply_timed(2.0, N.OP_NORMAL, x, y) for _ in range(max(5, args.steps // 10))]).mean(axis=0)
+    # error against the oracle (the checker) on the same matrix
+    y1 = torch.zeros_like(x)
+    plan.apply(1.0, N.OP_NORMAL, x, y1)
+    torch.cuda.synchronize()
+    yr = np.zeros(hm.n)
+    oracle.mul_vec(hm.blocks, hm.arena, 1.0, N.OP_NORMAL, x_h, yr)
+    err = float(np.linalg.norm(y1.cpu().numpy() - yr) / np.linalg.norm(yr))
+    cpu = None if args.no_cpu else cpu_baseline(hm.blocks, hm.arena, x_h, hm.n, args.cpu_seconds)
+    n = hm.n
+    r_bytes = info["bytes_rpass"] + 8.0 * (n + n)
+    achieved = r_bytes / (phases[2] * 1e-3) / 1e9
+    gbs = info["datasize"] / (ms * 1e-3) / 1e9
+    log(f"matern fp64: {ms:.4f} ms/MVM, phases {phases.round(4).tolist()} ms, {gbs:.0f} GB/s, rel.err {err:.2e}")
+    out = {
+        "metric": "H-MVM/s, plain FP64 drop-in (Matern n=65536) vs the CPU restatement",
+        "value": 1000.0 / ms, "unit": "H-MVM/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded points, generated H-matrix)",
+        "config": {"workload": f"Matern nu=1.5 l=0.1 on n={n} points in the unit cube, leaf 64, eta 2, "
+                               "ACA eps 1e-6, FP64, nrhs 1", "parallelism": "single GPU"},
+        "rel_err_vs_oracle": err, "bytes_per_mvm": info["datasize"], "hbm_gbs": gbs,
+        "hbm_frac": gbs / HBM_PEAK_GBS, "phases_ms": phases.tolist(),
+        "roofline": {"bound": "hbm", "kernel": "k_ufwd (row-panel pass, FP64)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": r_bytes},
+        "cpu_baseline": cpu,
+    }
+    if cpu is not None:
+        out["speedup_vs_cpu"] = out["value"] / cpu["value"]
+        out["speedup_vs_cpu_1core"] = out["value"] / cpu["value_1core"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_uniform(args) -> int:
+    """Uniform-H (shared bases) and H^2 (nested bases) forms of 1/r on
+    n = 65536 points on the sphere (leaf 64, eta 2, ACA eps 1e-6, bases cut
+    at eps 1e-6), FP64 and mixed-precision storage, beside the H-matrix they
+    were converted from.  One step is one y += 2 M x (hmvm_uplan_apply: the
+    chain of forward transform, couplings, backward transform and near-field
+    passes); the relative error is taken against the oracle's
+    uniform::mul_vec / h2::mul_vec on the same matrix, and that restatement
+    is timed on one host core beside it."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from hlr_amd import _native as N
+    from hlr_amd.hmatrix import INVR_SPHERE, HMatrix, random_vector
+    from hlr_amd.plan import Plan
+    from hlr_amd.uniform import UPlan, build_uniform
+
+    dev = torch.device("cuda:0")
+    n = args.n if args.n != 262144 else 65536
+    t0 = time.time()
+    hm = HMatrix(INVR_SPHERE.with_(n=n, eps=args.eps))
+    forms = {"uniform": build_uniform(hm, args.eps, nested=False), "h2": build_uniform(hm, args.eps, nested=True)}
+    log(f"sphere n={n}: H-matrix and its uniform-H / H^2 forms built in {time.time() - t0:.1f} s")
+    x_h = random_vector(n, 2)
+    x = torch.as_tensor(x_h, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    y = torch.zeros_like(x)
+    ph = Plan.from_hmatrix(hm, precision="fp64")
+    ws = torch.empty(ph.workspace_bytes(N.OP_NORMAL, 1) // 8 + 1, dtype=torch.float64, device=dev)
+    ms_h = timed(lambda: ph.apply_ws(2.0, N.OP_NORMAL, x, y, ws), args.steps, args.warmup)
+    h_bytes = ph.info()["datasize"]
+    ph.close()
+    del ws
+    res = {}
+    for name, um in forms.items():
+        yr = np.zeros(n)
+        oracle.uniform_mul_vec(um.rowcb, um.colcb, um.blocks, um.arena, 1.0, N.OP_NORMAL, x_h, yr)
+        for prec in ("fp64", "valr"):
+            up = UPlan(um, precision=prec, eps=args.eps)
+            info = up.info()
+            ms = timed(lambda: up.apply(2.0, N.OP_NORMAL, x, y), args.steps, args.warmup)
+            y1 = torch.zeros_like(x)
+            up.apply(1.0, N.OP_NORMAL, x, y1)
+            torch.cuda.synchronize()
+            err = float(np.linalg.norm(y1.cpu().numpy() - yr) / np.linalg.norm(yr))
+            res[f"{name}_{prec}"] = {"ms_per_mvm": ms, "mvm_per_s": 1000.0 / ms, "rel_err_vs_oracle_fp64": err,
+                                     "datasize_fp64": info["datasize"],
+                                     "hbm_gbs_fp64_bytes": info["datasize"] / (ms * 1e-3) / 1e9,
+                                     "levels": info["levels"]}
+            log(f"{name} {prec}: {ms:.4f} ms/MVM, {info['datasize'] / 1e6:.1f} MB (FP64), rel.err {err:.2e}")
+            up.close()
+    cpu = None
+    if not args.no_cpu:
+        um = forms["h2"]
+        yc = np.zeros(n)
+        t1 = time.time()
+        reps = 0
+        while time.time() - t1 < min(args.cpu_seconds, 10.0):
+            oracle.uniform_mul_vec(um.rowcb, um.colcb, um.blocks, um.arena, 2.0, N.OP_NORMAL, x_h, yc)
+            reps += 1
+        tc = time.time() - t1
+        cpu = {"value": reps / tc, "unit": "H-MVM/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} H^2 MVMs with the oracle's restatement of h2::mul_vec on one thread in {tc:.1f} s"}
+    best = res["h2_fp64"]
+    out = {
+        "metric": "H^2 / uniform-H MVM/s beside the H-matrix they come from (1/r sphere)",
+        "value": best["mvm_per_s"], "unit": "H-MVM/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms_per_mvm"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded points, generated H-matrix converted to shared / nested bases)",
+        "config": {"workload": f"1/r on unit sphere, n={n}, leaf 64, eta 2, ACA eps {args.eps:g}, "
+                               f"bases cut at eps {args.eps:g}; H^2 FP64 is the value", "parallelism": "single GPU"},
+        "h_matrix_fp64": {"ms_per_mvm": ms_h, "datasize": h_bytes},
+        "forms": res,
+        "roofline": {"bound": "hbm", "kernel": "whole H^2 product (chain of device passes)",
+                     "achieved": best["hbm_gbs_fp64_bytes"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": best["hbm_gbs_fp64_bytes"] / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": best["datasize_fp64"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_batch_baseline(hms, nrhs) -> dict:
+    """cpu_baseline leg of the batch: the reference applies one matrix and
+    one right-hand side at a time (programs/mvm/mvm.hh:151-204), so every
+    member is timed once with the oracle's restatement of tbb::mul_vec_chunk
+    on all cores of the affinity set (at most 16), and a step is the sum of
+    nrhs times that time over the members (one FP64 H-MVM sampled per member)."""
+    import numpy as np
+
+    import oracle
+    from oracle.build import CFLAGS
+    from hlr_amd.hmatrix import random_vector
+
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    step = 0.0
+    for i, (hm, r) in enumerate(zip(hms, nrhs)):
+        xv = random_vector(hm.n, 9000 + i)
+        t = oracle.time_chunk(hm.blocks, hm.arena, xv, np.zeros(hm.n), cores, 1)
+        step += r * t
+    products = float(sum(nrhs))
+    log(f"cpu baseline (batch): {step:.2f} s per step on {cores} threads")
+    return {"value": products / step, "unit": "H-MVM/s", "cores": cores, "kind": "port",
+            "sample": (f"one FP64 H-MVM per member ({len(hms)}) with the oracle's restatement of tbb::mul_vec_chunk on "
+                       f"{cores} threads, scaled by each member's nrhs: {step:.2f} s per step; "
+                       f"gcc {' '.join(CFLAGS)}")}
+
+
+def cpu_baseline(blocks, arena, x_h, n: int, seconds: float) -> dict:
+    """cpu_baseline leg: the oracle's restatement of tbb::mul_vec_chunk
+    (parallel leaf loop, 64-entry locked chunks of y) timed on the host, on
+    all cores of the affinity set (at most 16) and on one core, each on a
+    bounded sample of about `seconds` of work.  The oracle is the checker;
+    here it is only timed."""
+    import numpy as np
+
+    import oracle
+    from oracle.build import CFLAGS
+
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    y = np.zeros(n)
+    rate, sample = {}, []
+    for nt in (cores, 1):
+        # calibrate on doubling samples of at least 0.2 s, then time the bounded sample
+        nm, t1 = 1, oracle.time_chunk(blocks, arena, x_h, y, nt, 1)
+        while t1 < 0.2 and nm < 1 << 20:
+            nm *= 2
+            t1 = oracle.time_chunk(blocks, arena, x_h, y, nt, nm)
+        nmvm = int(seconds * nm / max(t1, 1e-9))
+        if nmvm > nm:
+            tc = oracle.time_chunk(blocks, arena, x_h, y, nt, nmvm)
+        else:
+            nmvm, tc = nm, t1
+        rate[nt] = nmvm / tc
+        sample.append(f"{nmvm} MVMs in {tc:.1f} s on {nt} thread{'s' if nt > 1 else ''}")
+        log(f"cpu baseline: {rate[nt]:.3f} MVM/s on {nt} thread(s)")
+    return {"value": rate[cores], "unit": "H-MVM/s", "cores": cores, "kind": "port",
+            "value_1core": rate[1],
+            "sample": (f"FP64 H-MVMs of the same matrix with the oracle's restatement of tbb::mul_vec_chunk "
+                       f"(64-entry locked chunks): {'; '.join(sample)}; gcc {' '.join(CFLAGS)}; {model}")}
 
 
 def bench_grad(args) -> int:
@@ -486,6 +722,7 @@ def bench_batch(args) -> int:
     ms_graph = timed(lambda: b.replay(stream=gs), args.steps, args.warmup, st=gs)
     b.check(stream=gs)
     products = float(sum(nrhs))
+    cpu = None if args.no_cpu else cpu_batch_baseline(hms, nrhs)
     # factor bytes once per member and call, vectors per right-hand side
     nbytes = float(sum(i["datasize"] + 8.0 * 2 * (r - 1) * (i["nrows"] + i["ncols"]) for i, r in zip(infos, nrhs)))
     flops = float(sum(i["flops"] * r for i, r in zip(infos, nrhs)))
@@ -510,7 +747,7 @@ def bench_batch(args) -> int:
                      "achieved": nbytes / (ms_batch * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": nbytes / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": nbytes},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     b.close()

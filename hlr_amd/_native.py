@@ -200,6 +200,13 @@ _hlrgen = None
 def hmvm() -> C.CDLL:
     global _hmvm
     if _hmvm is None:
+        # torch ships its own HIP runtime under the same soname: load it first,
+        # so that the library binds to it instead of loading a second runtime
+        # (two runtimes in one process see no devices in one of them)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _hmvm = _load("libhmvm.so", HMVM_SYMBOLS)
     return _hmvm
 
