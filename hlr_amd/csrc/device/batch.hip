@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_bv1(const BatchMat* __restrict__ m
     if (wave >= w.count) return;
     const VPack* pk = M.D.vpacks + w.first + wave;
     const XT* x = static_cast<const XT*>(M.x);
-    double* xs = lds + wave * kPackSegs * kXsStride;
+    double* xs = lds + wave * kPackXs;
     HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(M.D, pk, x, M.part, xs, lane))
 }
 
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kThreads) void k_bvm(const BatchMat* __restrict__ m
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
-    vseg_mm_dispatch<NTM, FAM>(M.D, M.D.vitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
+    vseg_mm_dispatch<NTM, FAM>(M.D, M.D.fitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
 }
 
 // partial sums of one member: items (entry, pair of right-hand sides) when R
@@ -328,14 +328,14 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
                     double by = 0;
                     for (int j = 0; j < cnt; ++j) {
                         const VPack& pk = L.vpacks[k + j];
-                        by += double(pk.gofs[pk.nseg]) * 64.0 * 16.0;
+                        for (int t = 0; t < pk.nseg; ++t) by += double(pk.ng[t]) * double(pk.m[t]) * 16.0;
                     }
                     K[K_V1].push_back({{i, int32_t(k), cnt, 0}, by});
                 }
             } else {
-                for (int64_t k = 0; k < D.nvitems; ++k)
+                for (int64_t k = 0; k < D.nfitems; ++k)
                     K[K_VM1 + (R + 15) / 16 - 1].push_back(
-                        {{i, int32_t(k), 1, 0}, double(L.vitems[k].ng) * 16.0 * 64.0 * R});
+                        {{i, int32_t(k), 1, 0}, double(L.fitems[k].ng) * 16.0 * double(L.fitems[k].m) * R});
                 for (int64_t i0 = 0; i0 < D.ncols; i0 += 64)
                     K[K_XT].push_back({{i, int32_t(i0 / 64), 1, 0}, double(std::min<int64_t>(64, D.ncols - i0)) * R});
             }
@@ -368,7 +368,7 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
         }
     }
     b->lds[K_XT] = mm_lds_bytes(maxRp);
-    b->lds[K_V1] = size_t(kWaves) * kPackSegs * kXsStride * sizeof(double);
+    b->lds[K_V1] = size_t(kWaves) * kPackXs * sizeof(double);
     for (int k = K_VM1; k <= K_VM4; ++k) b->lds[k] = mm_lds_bytes(maxRp);
     b->lds[K_R1] = size_t(kChunkGroups) * 16 * sizeof(float);
     for (int k = K_RM1; k <= K_RM4; ++k) b->lds[k] = mm_lds_bytes(maxRp);

@@ -10,7 +10,6 @@ namespace hmvm {
 
 constexpr int kThreads = 64 * kWaves;
 constexpr int64_t kWholeZBytes = 48 * 1024;   // largest panel z block staged whole in LDS
-constexpr int kXsStride = 65;                 // LDS row stride of the x slices of a V pack
 
 // ---------------------------------------------------------------------------
 // V pass: one wave processes one pack of consecutive segments (lane = group)
@@ -23,14 +22,15 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     using A = typename E::Acc;
     constexpr int W = E::W;
     const VPack& pk = *pkp;     // read through the pointer: the arrays stay out of scratch
-    // x slices of the segments, row stride kXsStride (odd): the lanes of
-    // different segments read different rows i of their own slice in the
-    // same step, which then fall into different LDS banks
-    A* xs = static_cast<A*>(xsraw);                 // [kPackSegs][kXsStride]
+    // x slices of the segments (up to kUnitRows values each), an odd
+    // distance apart: the lanes of different segments read different rows i
+    // of their own slice in the same step, which then fall into different
+    // LDS banks
+    A* xs = static_cast<A*>(xsraw);                 // kPackXs values: segment j's slice at xoff[j]
     const int nseg = pk.nseg;
-    for (int idx = lane; idx < nseg * 64; idx += 64) {
-        const int j = idx >> 6, i = idx & 63;
-        if (i < pk.m[j]) xs[j * kXsStride + i] = A(x[pk.x0[j] + i]);
+    for (int j = 0; j < nseg; ++j) {
+        const int mj = pk.m[j], o = pk.xoff[j], x0 = pk.x0[j];
+        for (int i = lane; i < mj; i += 64) xs[o + i] = A(x[x0 + i]);
     }
     wave_sync();
     const int ngr = pk.gofs[nseg];                  // groups of the pack = active lanes
@@ -40,7 +40,7 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     const int m = (lane < ngr) ? pk.m[j] : 0;       // lanes past the pack's groups load nothing
     const int64_t ldb = int64_t(pk.ng[j]) * 16;                 // bytes per row of the segment
     const bytep base = elem_addr<T>(P.arena[E::F], pk.data_off[j]) + int64_t(g) * 16;
-    const A* xj = xs + j * kXsStride;
+    const A* xj = xs + pk.xoff[j];
     A acc[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) acc[w] = A(0);
@@ -58,7 +58,7 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     // consecutive lanes then handle consecutive columns: the destination
     // and scale tables are read and the partials written coalesced, instead
     // of W strided accesses per lane (one cache line per lane for wide W).
-    static_assert(64 * W * sizeof(A) <= kPackSegs * kXsStride * sizeof(double), "result staging fits the slices");
+    static_assert(64 * W * sizeof(A) <= kPackXs * sizeof(double), "result staging fits the slices");
     wave_sync();
     A* res = xs;
 #pragma unroll
@@ -75,7 +75,7 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     for (int k = 0; k < W; ++k) {
         const int c = lane + 64 * k;
         const int gr = c / W;
-        int32_t vb = pk.vc0[0] - pk.gofs[0] * W;
+        int64_t vb = pk.vc0[0] - pk.gofs[0] * W;
 #pragma unroll
         for (int t = 1; t < kPackSegs; ++t)
             if (t < nseg && gr >= pk.gofs[t]) vb = pk.vc0[t] - pk.gofs[t] * W;

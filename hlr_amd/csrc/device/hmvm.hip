@@ -46,7 +46,7 @@ namespace {
 // stream was measured at
 template <typename XT, int FAM>
 __global__ __launch_bounds__(kThreads, 4) void k_vfwd(DevPlan P, const XT* __restrict__ x, double* __restrict__ Pb) {
-    __shared__ double xs[kWaves][kPackSegs * kXsStride];
+    __shared__ double xs[kWaves][kPackXs];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t pi = int64_t(blockIdx.x) * kWaves + wave;
     if (pi >= P.nvpacks) return;
@@ -227,7 +227,7 @@ __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VIte
     A sv[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        const int64_t vc = C.vcol0[E::F] + int64_t(it.g0 + g) * W + w;
+        const int64_t vc = it.vc0 + int64_t(g) * W + w;
         const int32_t e = on ? P.vsrc[vc] : -1;
         double v = (e >= 0) ? s[e] : 0.0;
         if (E::kScaled && on) v *= double(P.vscale[vc]);
@@ -354,18 +354,17 @@ struct GradPtrs {
 template <int W, typename XT>
 __device__ inline void grad_v_group(const DevPlan& P, const VItem& it, const XT* __restrict__ x,
                                     const double* __restrict__ s, double* __restrict__ gV, int lane) {
-    const ColPanel* C = P.cp + it.q;                     // read in place: vcol0 is indexed by the format
     const int64_t ld = int64_t(it.ng) * W;
     double sv[W];
-    const int64_t vc0 = C->vcol0[it.fmt] + int64_t(it.g0 + lane) * W;
+    const int64_t vc0 = it.vc0 + int64_t(lane) * W;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         const int32_t e = P.vsrc[vc0 + w];
         sv[w] = (e >= 0) ? s[e] : 0.0;
     }
     double* dst = gV + it.data_off + int64_t(lane) * W;
-    const int m = C->m;
-    const int64_t col0 = C->col0;
+    const int m = it.m;
+    const int64_t col0 = it.x0;
     for (int i = 0; i < m; ++i) {
         const double xi = double(x[col0 + i]);
         double* d = dst + int64_t(i) * ld;
@@ -462,6 +461,7 @@ int upload_plan(hmvm_plan* p) {
     UP(qd_cnt, L.qd_cnt);
     UP(tstride, L.tstride);
     UP(vitems, L.vitems);
+    UP(fitems, L.fitems);
     UP(vpacks, L.vpacks);
     UP(pl, L.pl);
     UP(pl_il, L.pl_il);
@@ -491,6 +491,7 @@ int upload_plan(hmvm_plan* p) {
     D.nrp = int64_t(L.rp.size());
     D.ncp = int64_t(L.cp.size());
     D.nvitems = int64_t(L.vitems.size());
+    D.nfitems = int64_t(L.fitems.size());
     D.nvpacks = int64_t(L.vpacks.size());
     D.nrchunks = int64_t(L.rchunks.size());
     D.ntent = L.ntent;

@@ -21,7 +21,8 @@ struct DevPlan {
     const RowPanel* rp;
     const RChunk* rchunks;
     const ColPanel* cp;
-    const VItem* vitems;
+    const VItem* vitems;      // panel views of the V segments
+    const VItem* fitems;      // V segments
     const int32_t* colsrc;
     const float* zscale;      // null: all 1
     const int32_t* rdest;
@@ -44,7 +45,7 @@ struct DevPlan {
     const int32_t* qd_off;
     const int32_t* qd_cnt;
     const void* arena[NFMT];
-    int64_t nrp, ncp, nvitems, nrchunks, ntent, psize, qsize, nrows, ncols;
+    int64_t nrp, ncp, nvitems, nfitems, nrchunks, ntent, psize, qsize, nrows, ncols;
 };
 
 // ---------------------------------------------------------------------------

@@ -39,13 +39,13 @@ __global__ __launch_bounds__(kThr) void k_xtrans(const XT* __restrict__ X, int64
     xtrans_tile(X, ldx, n, R, XTb, int64_t(blockIdx.x) * 64, tile, threadIdx.x);
 }
 
-// one workgroup per V segment (x slice 64 x Rp in dynamic LDS)
+// one workgroup per V segment (x slices of 64 x Rp in dynamic LDS)
 template <int NTM, int FAM>
 __global__ __launch_bounds__(kThr) void k_vfwd_mm(DevPlan P, const double* __restrict__ XTb, int R,
                                                   double* __restrict__ Pb) {
     extern __shared__ double xs[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const VItem sg = P.vitems[blockIdx.x];
+    const VItem sg = P.fitems[blockIdx.x];
     vseg_mm_dispatch<NTM, FAM>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
 }
 
@@ -96,9 +96,9 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
     // own formats' decode paths (and registers)
     auto vpass = [&](auto ntm) {
         constexpr int NTM = decltype(ntm)::value;
-        if (fam == kFamHw) k_vfwd_mm<NTM, kFamHw><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
-        else if (fam == kFamF64) k_vfwd_mm<NTM, kFamF64><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
-        else k_vfwd_mm<NTM, kFamBfp><<<unsigned(D.nvitems), kThr, lds, st>>>(D, xt, R, part);
+        if (fam == kFamHw) k_vfwd_mm<NTM, kFamHw><<<unsigned(D.nfitems), kThr, lds, st>>>(D, xt, R, part);
+        else if (fam == kFamF64) k_vfwd_mm<NTM, kFamF64><<<unsigned(D.nfitems), kThr, lds, st>>>(D, xt, R, part);
+        else k_vfwd_mm<NTM, kFamBfp><<<unsigned(D.nfitems), kThr, lds, st>>>(D, xt, R, part);
     };
     auto rpass = [&](auto ntm) {
         constexpr int NTM = decltype(ntm)::value;
@@ -123,7 +123,7 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
         default: pass(std::integral_constant<int, 4>()); break;
         }
     };
-    if (D.nvitems > 0) by_tiles(vpass);
+    if (D.nfitems > 0) by_tiles(vpass);
     if (D.ntent > 0) {
         // pairs of right-hand sides when R is even and the workspace 16-byte aligned
         if (R % 2 == 0 && (reinterpret_cast<uintptr_t>(part) & 15) == 0)

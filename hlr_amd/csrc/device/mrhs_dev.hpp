@@ -93,51 +93,66 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
     constexpr int W = M::W;
     constexpr int JB = mm_jb(W);
     constexpr int NB = MmSteps<W>::value;
-    const ColPanel C = P.cp[sg.q];
-    const int m = C.m;
+    const int m = sg.m;                                  // up to kUnitRows, staged 64 rows at a time
     const int Rp = (R + 15) & ~15;
     const int NT = Rp / 16;
     constexpr int XS = kMmStride;
     A* xs = reinterpret_cast<A*>(xsraw);                 // [64][XS]: row, rhs
     constexpr int RS = 16 * NTM;                         // staged rhs slots (constant divisor)
-    for (int idx = tid; idx < 64 * RS; idx += kThr) {
-        const int i = idx / RS, r = idx - i * RS;         // rhs fastest: contiguous reads of XT
-        if (r < Rp) xs[i * XS + r] = (i < m && r < R) ? A(XTb[(C.col0 + i) * R + r]) : A(0);
-    }
+    // rows i0 .. i0 + 63 of the segment's x slice
+    auto stage = [&](int i0) {
+        const int mb = min(64, m - i0);
+        for (int idx = tid; idx < 64 * RS; idx += kThr) {
+            const int i = idx / RS, r = idx - i * RS;     // rhs fastest: contiguous reads of XT
+            if (r < Rp) xs[i * XS + r] = (i < mb && r < R) ? A(XTb[(sg.x0 + i0 + i) * R + r]) : A(0);
+        }
+    };
+    const bool single = m <= 64;                         // one block: staged once for every pass
+    if (single) stage(0);
     __syncthreads();
-    if (16 * wave >= sg.ng) return;                      // no barrier follows
+    // waves past the segment's groups keep taking part in the staging barriers
+    const bool wactive = 16 * wave < sg.ng;
     const int n = lane & 15, kk = lane >> 4;
     const int gl = 16 * wave + n;
     const bool gon = gl < sg.ng;
     const int64_t ldb = int64_t(sg.ng) * 16;                // bytes per row of the segment
     const bytep base = elem_addr<T>(P.arena[M::F], sg.data_off) + int64_t(gon ? gl : 0) * 16;
-    const int64_t vc0 = C.vcol0[M::F] + int64_t(sg.g0) * W;
+    const int64_t vc0 = sg.vc0;
     for (int jp = 0; jp < W; jp += JB) {
         typename M::V4 acc[JB][NTM];
 #pragma unroll
         for (int j = 0; j < JB; ++j)
 #pragma unroll
             for (int nt = 0; nt < NTM; ++nt) acc[j][nt] = typename M::V4{0, 0, 0, 0};
-        // NB steps of 4 rows: their loads are all in flight before the MFMAs
-        for (int s0 = 0; 4 * s0 < m; s0 += NB) {
-            u32x4 raw[NB];
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int row = 4 * (s0 + b) + kk;
-                raw[b] = (gon && row < m) ? raw16(base + int64_t(row) * ldb) : u32x4{0u, 0u, 0u, 0u};
+        for (int i0 = 0; i0 < m; i0 += 64) {
+            if (!single) {
+                __syncthreads();                         // the previous block's reads are done
+                stage(i0);
+                __syncthreads();
             }
+            const int mb = min(64, m - i0);
+            if (!wactive) continue;
+            // NB steps of 4 rows: their loads are all in flight before the MFMAs
+            for (int s0 = 0; 4 * s0 < mb; s0 += NB) {
+                u32x4 raw[NB];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (4 * (s0 + b) >= m) break;            // uniform
-                const int row = 4 * (s0 + b) + kk;
-                A v[W];
-                E::decode(raw[b], v);
+                for (int b = 0; b < NB; ++b) {
+                    const int row = 4 * (s0 + b) + kk;
+                    raw[b] = (gon && row < mb) ? raw16(base + int64_t(i0 + row) * ldb) : u32x4{0u, 0u, 0u, 0u};
+                }
 #pragma unroll
-                for (int nt = 0; nt < NTM; ++nt) {
-                    if (nt < NT) {
-                        const A a = xs[row * XS + nt * 16 + n];
+                for (int b = 0; b < NB; ++b) {
+                    if (4 * (s0 + b) >= mb) break;       // uniform
+                    const int row = 4 * (s0 + b) + kk;
+                    A v[W];
+                    E::decode(raw[b], v);
 #pragma unroll
-                        for (int j = 0; j < JB; ++j) acc[j][nt] = M::mma(a, v[jp + j], acc[j][nt]);
+                    for (int nt = 0; nt < NTM; ++nt) {
+                        if (nt < NT) {
+                            const A a = xs[row * XS + nt * 16 + n];
+#pragma unroll
+                            for (int j = 0; j < JB; ++j) acc[j][nt] = M::mma(a, v[jp + j], acc[j][nt]);
+                        }
                     }
                 }
             }

@@ -4,21 +4,24 @@
 // (one wavefront).  Every leaf contributes
 //   - to each row panel it covers: its dense rows D[p,:] (width n) or its U
 //     rows U[p,:] (width k)                                     ("R arena");
-//   - to each column panel it covers (low-rank only): its V rows V[q,:]
-//     (width k)                                                 ("V arena").
-// Columns inside a panel are grouped by storage format and cut into groups
-// of W columns, W * sizeof(element) == 16 bytes, so that a lane loads W
-// values of one row with a single 16-byte load.
+//   - (low-rank only) its V rows, grouped by the column index set the leaf
+//     shares with others and cut into units of up to four column panels
+//     (256 rows)                                                ("V arena").
+// Columns are grouped by storage format and cut into groups of W columns,
+// W * sizeof(element) == 16 bytes, so that a lane loads W values of one row
+// with a single 16-byte load.
 //   R: a chunk of up to kChunkGroups groups of one format is contiguous;
 //      group g of the chunk is an m x W row-major block (lane = row).
-//   V: an item of up to 64 groups of one format is contiguous; it is an
-//      m x (ng*W) row-major block (lane = group, rows streamed).
-// Every chunk and item is one contiguous stream of HBM.
+//   V: a segment of up to 64 groups of one format of one unit is
+//      contiguous; it is an m x (ng*W) row-major block (lane = group, rows
+//      streamed); its rows of one column panel are that panel's view.
+// Every chunk and segment is one contiguous stream of HBM.
 //
-// Forward  y += a A x :  P = V^T x per column panel (no reduction),
-//                        t = a * sum_q P,  y_p += R_p z  (z = a x | t).
+// Forward  y += a A x :  P = V^T x per unit (no reduction),
+//                        t = a * sum_u P,  y_p += R_p z  (z = a x | t).
 // Transposed y += a A^T x: Q = R_p^T x_p per row panel (lane reduction),
-//                        s = a * sum_p Q,  y_q += V_q s + a * sum Q_dense.
+//                        s = a * sum_p Q,  y_q += V_q s + a * sum Q_dense
+//                        (V_q: the panel views of the segments).
 #pragma once
 #include <cstdint>
 
@@ -95,21 +98,31 @@ struct RChunk {         // up to kChunkGroups groups of one format of one row pa
 struct ColPanel {
     int64_t col0;       // first global column
     int32_t m;          // rows of V in this panel (<= 64)
-    int32_t nitems;     // V items of this panel
-    int64_t item0;      // first item in the item table
-    int32_t kpad[NFMT]; // padded width per format (multiple of W)
-    int64_t vcol0[NFMT];// first column in the per-V-column tables
+    int32_t nitems;     // panel views of V segments (VItem) of this panel
+    int64_t item0;      // first view in the panel view table
 };
 
 constexpr int kPackSegs = 8;        // V segments per forward pack
 constexpr int kSegLen = 16;         // partial sums per thread in the first summation stage
+// A V segment covers one unit of a column cluster: up to kUnitPanels
+// consecutive column panels (at most kUnitRows rows) of the column index set
+// shared by its blocks; a low-rank block has one forward partial per unit.
+constexpr int kUnitPanels = 4;
+constexpr int kUnitRows = kUnitPanels * kPanel;
+constexpr int kPackXs = kPackSegs * 65;      // LDS values of a pack's x slices (per wave)
 
-struct VItem {          // V segment: up to kItemGroups groups of one format of one column panel
-    int32_t q;
+// V segment (forward table) or its view in one column panel (panel table):
+// an m x (ng W) row-major block of one format, lane = group, row stride ng W
+// values; its columns are V columns vc0 .. vc0 + ng W - 1 of the per-V-column
+// tables, for rows x0 .. x0 + m - 1 of the vectors
+struct VItem {
+    int32_t q;          // panel views: column panel; segments: first panel of the unit
     int32_t fmt;
-    int32_t g0;         // first group inside the panel's format block
     int32_t ng;
-    int64_t data_off;   // element offset of the m x (ng*W) row-major block in arena[fmt]
+    int32_t m;          // rows (segments <= kUnitRows, panel views <= kPanel)
+    int64_t data_off;   // element offset of the block's first row in arena[fmt]
+    int64_t vc0;        // first V column
+    int64_t x0;         // first row's global index
 };
 
 // one lane of the single right-hand-side partial sums: entry e (-1:
@@ -130,10 +143,11 @@ struct VPack {          // forward V work unit of one wave: consecutive segments
     int32_t fmt;        // at most 64 groups in total (lane = group); everything a lane
     int32_t nseg;       // needs is stored here, so the pack starts without load chains
     int32_t gofs[kPackSegs + 1];   // prefix sums of the segments' group counts
-    int32_t m[kPackSegs];          // rows of each segment's column panel
+    int32_t m[kPackSegs];          // rows of each segment (<= kUnitRows)
     int32_t ng[kPackSegs];         // groups of each segment
-    int32_t x0[kPackSegs];         // first x index (column panel start)
-    int32_t vc0[kPackSegs];        // per-V-column table index of each segment's group 0
+    int32_t xoff[kPackSegs];       // first value of each segment's x slice in the wave's LDS
+    int32_t x0[kPackSegs];         // first x index of each segment
+    int64_t vc0[kPackSegs];        // per-V-column table index of each segment's column 0
     int64_t data_off[kPackSegs];   // element offset of each segment's block
 };
 

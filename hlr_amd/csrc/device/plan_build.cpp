@@ -24,6 +24,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <omp.h>
 
@@ -317,6 +318,22 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         bq1[i] = find_panel(cstart, b.col_ofs + b.ncols);
         if (b.nrows == 0 || b.ncols == 0) { bp1[i] = bp0[i]; bq1[i] = bq0[i]; }
     }
+    // units of a low-rank block: its column panels in runs of at most
+    // kUnitPanels from its first panel (at most kUnitRows rows); blocks with
+    // the same column index set share their units, and their V segments of
+    // the wide formats (FP64, FP32: W <= 4).  Columns of the narrow formats
+    // (16-byte rows of 5 to 16 values) stay in one segment per column panel
+    // and format across all blocks of the panel, so that a segment still
+    // fills a wave's lanes.
+    auto wide = [](int f) { return fmt_width(f) <= 4; };
+    std::vector<int64_t> nunit(nb, 0);
+    for (int64_t i = 0; i < nb; ++i)
+        if (blocks[i].kind != HMVM_BLOCK_DENSE && blocks[i].rank > 0)
+            nunit[i] = (bq1[i] - bq0[i] + kUnitPanels - 1) / kUnitPanels;
+    auto unit_panels = [&](int64_t i, int64_t u) {
+        const int64_t a = bq0[i] + u * kUnitPanels;
+        return std::make_pair(a, std::min<int64_t>(a + kUnitPanels, bq1[i]));
+    };
 
     // ---- t entries and their formats ----
     L.blk_t0.assign(nb, -1);
@@ -405,7 +422,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     std::vector<std::vector<Seg>> rseg(NR), cseg(NC);
     for (int64_t i = 0; i < nb; ++i) {
         for (int64_t p = bp0[i]; p < bp1[i]; ++p) rseg[p].push_back({i, int32_t(p - bp0[i])});
-        if (blocks[i].kind != HMVM_BLOCK_DENSE && blocks[i].rank > 0)
+        if (nunit[i] > 0)
             for (int64_t q = bq0[i]; q < bq1[i]; ++q) cseg[q].push_back({i, int32_t(q - bq0[i])});
     }
     {
@@ -424,6 +441,48 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         if (bad) return HMVM_ERR_OVERLAP;
     }
 
+    // column index sets of the low-rank blocks; a set's columns of a wide
+    // format go to tall segments (one per unit) when they fill at least half
+    // a wave (kItemGroups / 2 groups), else to the per-panel segments
+    // column index sets of the low-rank blocks, in order of first column
+    struct ColSet {
+        int64_t b0;                    // a block of the set (its panels and units)
+        std::vector<int64_t> blocks;   // blocks of the set, in block order
+        int64_t unit0 = 0;             // first unit of the set in the unit table
+    };
+    std::vector<ColSet> csets;
+    {
+        std::map<std::pair<int64_t, int64_t>, int64_t> id;
+        for (int64_t i = 0; i < nb; ++i) {
+            if (nunit[i] == 0) continue;
+            auto key = std::make_pair(int64_t(blocks[i].col_ofs), int64_t(blocks[i].ncols));
+            auto it = id.find(key);
+            if (it == id.end()) it = id.emplace(key, int64_t(-1)).first;
+            (void)it;
+        }
+        int64_t k = 0;
+        for (auto& kv : id) kv.second = k++;
+        csets.resize(size_t(k));
+        for (int64_t i = 0; i < nb; ++i) {
+            if (nunit[i] == 0) continue;
+            ColSet& cs = csets[size_t(id[std::make_pair(int64_t(blocks[i].col_ofs), int64_t(blocks[i].ncols))])];
+            if (cs.blocks.empty()) cs.b0 = i;
+            cs.blocks.push_back(i);
+        }
+    }
+    std::vector<int64_t> blk_set(nb, -1);
+    std::vector<std::array<char, NFMT>> settall(csets.size());
+    for (size_t c = 0; c < csets.size(); ++c) {
+        int64_t ncol[NFMT] = {};
+        for (int64_t i : csets[c].blocks) {
+            blk_set[i] = int64_t(c);
+            for (int64_t l = 0; l < blocks[i].rank; ++l) ncol[L.tfmt[L.blk_t0[i] + l]]++;
+        }
+        for (int f = 0; f < NFMT; ++f)
+            settall[c][f] = wide(f) && round_up(ncol[f], fmt_width(f)) / fmt_width(f) >= kItemGroups / 2;
+    }
+    auto tall = [&](int64_t i, int f) { return blk_set[i] >= 0 && settall[size_t(blk_set[i])][f] != 0; };
+
     // ---- forward partials P and transposed partials Q, block-major:
     //      entry (panel index r, column l) of block b at base_b + r * k_b + l,
     //      so the sums over r read consecutive l with unit stride ----
@@ -437,7 +496,8 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             const int64_t k = blocks[i].rank;
             for (int64_t l = 0; l < k; ++l) {
                 const int64_t e = L.blk_t0[i] + l;
-                L.pt_off[e] = int32_t(pos + l); L.pt_cnt[e] = int32_t(bq1[i] - bq0[i]);
+                L.pt_off[e] = int32_t(pos + l);
+                L.pt_cnt[e] = int32_t(tall(i, L.tfmt[e]) ? nunit[i] : bq1[i] - bq0[i]);
                 L.qt_off[e] = int32_t(qpos + l); L.qt_cnt[e] = int32_t(bp1[i] - bp0[i]);
                 L.tstride[e] = int32_t(k);
             }
@@ -551,70 +611,133 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     }
     const int64_t nrcols = colpos;
 
-    // ---- column panels ----
-    L.cp.resize(NC);
-    std::vector<std::array<int64_t, NFMT>> ccols(NC);
-    for (int64_t q = 0; q < NC; ++q) {
-        std::array<int64_t, NFMT> c{};
-        for (const Seg& s : cseg[q])
-            for (int64_t l = 0; l < blocks[s.b].rank; ++l) c[L.tfmt[L.blk_t0[s.b] + l]]++;
-        ccols[q] = c;
-    }
+    // ---- V segments: per column index set (its blocks), unit and format ----
+    int64_t nunits = 0;
+    for (ColSet& cs : csets) { cs.unit0 = nunits; nunits += nunit[cs.b0]; }
+    // first segment of each (unit, format); -1 when the unit has none
+    std::vector<std::array<int64_t, NFMT>> useg0(static_cast<size_t>(nunits));
     int64_t vpos = 0;
     int64_t varena_pos[NFMT];
     for (int f = 0; f < NFMT; ++f) varena_pos[f] = arena_pos[f];
-    for (int64_t q = 0; q < NC; ++q) {
-        ColPanel& C = L.cp[q];
-        C.col0 = cpan[q].first;
-        C.m = cpan[q].second;
-        C.item0 = int64_t(L.vitems.size());
-        for (int f = 0; f < NFMT; ++f) {
-            const int W = fmt_width(f);
-            const int64_t kp = round_up(ccols[q][f], W);
-            C.kpad[f] = int32_t(kp);
-            C.vcol0[f] = vpos;
-            vpos += kp;
-            const int64_t ng = kp / W;
-            // segments of kItemGroups groups (one lane each) plus a remainder
-            // that the forward packs merge with remainders of other panels;
-            // each segment is a contiguous m x (ng * W) row-major block that
-            // starts on a 128-byte boundary
-            const int64_t align = 8 * int64_t(W);          // values in 128 bytes
-            for (int64_t g = 0; g < ng; g += kItemGroups) {
-                const int64_t cnt = std::min<int64_t>(kItemGroups, ng - g);
-                varena_pos[f] = round_up(varena_pos[f], align);
-                VItem v;
-                v.q = int32_t(q); v.fmt = int32_t(f); v.g0 = int32_t(g); v.ng = int32_t(cnt);
-                v.data_off = varena_pos[f];
-                varena_pos[f] += int64_t(C.m) * cnt * W;
-                L.vitems.push_back(v);
+    L.fitems.clear();
+    for (const ColSet& cs : csets) {
+        for (int64_t u = 0; u < nunit[cs.b0]; ++u) {
+            const auto pq = unit_panels(cs.b0, u);
+            const int64_t ux0 = cstart[pq.first], um = cstart[pq.second] - cstart[pq.first];
+            int64_t ncol[NFMT] = {};
+            for (int64_t i : cs.blocks)
+                for (int64_t l = 0; l < blocks[i].rank; ++l) ncol[L.tfmt[L.blk_t0[i] + l]]++;
+            for (int f = 0; f < NFMT; ++f) {
+                useg0[size_t(cs.unit0 + u)][f] = -1;
+                if (ncol[f] == 0 || !tall(cs.b0, f)) continue;
+                const int W = fmt_width(f);
+                const int64_t ng = round_up(ncol[f], W) / W;
+                useg0[size_t(cs.unit0 + u)][f] = int64_t(L.fitems.size());
+                // segments of kItemGroups groups, each a contiguous um x (ng W)
+                // row-major block starting on a 128-byte boundary
+                for (int64_t g = 0; g < ng; g += kItemGroups) {
+                    VItem v;
+                    v.q = int32_t(pq.first);
+                    v.fmt = f;
+                    v.ng = int32_t(std::min<int64_t>(kItemGroups, ng - g));
+                    v.m = int32_t(um);
+                    varena_pos[f] = round_up(varena_pos[f], 8 * int64_t(W));
+                    v.data_off = varena_pos[f];
+                    v.vc0 = vpos;
+                    v.x0 = ux0;
+                    varena_pos[f] += um * int64_t(v.ng) * W;
+                    vpos += int64_t(v.ng) * W;
+                    L.fitems.push_back(v);
+                }
             }
         }
-        C.nitems = int32_t(int64_t(L.vitems.size()) - C.item0);
-        L.max_vitem_rows = std::max<int32_t>(L.max_vitem_rows, C.m);
     }
-    // forward packs: consecutive segments of one format, <= 64 groups per pack
+    // narrow formats: one run of segments per column panel and format, over
+    // all blocks of the panel
+    std::vector<std::array<int64_t, NFMT>> qseg0(static_cast<size_t>(NC));
+    for (int64_t q = 0; q < NC; ++q) {
+        int64_t ncol[NFMT] = {};
+        for (const Seg& sg : cseg[q])
+            for (int64_t l = 0; l < blocks[sg.b].rank; ++l) {
+                const int f = L.tfmt[L.blk_t0[sg.b] + l];
+                if (!tall(sg.b, f)) ncol[f]++;
+            }
+        for (int f = 0; f < NFMT; ++f) {
+            qseg0[size_t(q)][f] = -1;
+            if (ncol[f] == 0) continue;
+            const int W = fmt_width(f);
+            const int64_t ng = round_up(ncol[f], W) / W;
+            qseg0[size_t(q)][f] = int64_t(L.fitems.size());
+            for (int64_t g = 0; g < ng; g += kItemGroups) {
+                VItem v;
+                v.q = int32_t(q);
+                v.fmt = f;
+                v.ng = int32_t(std::min<int64_t>(kItemGroups, ng - g));
+                v.m = cpan[q].second;
+                varena_pos[f] = round_up(varena_pos[f], 8 * int64_t(W));
+                v.data_off = varena_pos[f];
+                v.vc0 = vpos;
+                v.x0 = cstart[q];
+                varena_pos[f] += int64_t(v.m) * v.ng * W;
+                vpos += int64_t(v.ng) * W;
+                L.fitems.push_back(v);
+            }
+        }
+    }
+    // per column panel: views of the segments covering it
+    L.cp.resize(NC);
+    {
+        std::vector<std::vector<VItem>> views(NC);
+        for (const VItem& sg : L.fitems)
+            for (int64_t q = sg.q; q < NC && cstart[q] < sg.x0 + sg.m; ++q) {
+                VItem v = sg;
+                v.q = int32_t(q);
+                v.m = cpan[q].second;
+                v.x0 = cstart[q];
+                v.data_off = sg.data_off + (cstart[q] - sg.x0) * int64_t(sg.ng) * fmt_width(sg.fmt);
+                views[q].push_back(v);
+            }
+        L.vitems.clear();
+        for (int64_t q = 0; q < NC; ++q) {
+            ColPanel& C = L.cp[q];
+            C.col0 = cpan[q].first;
+            C.m = cpan[q].second;
+            C.item0 = int64_t(L.vitems.size());
+            C.nitems = int32_t(views[q].size());
+            L.vitems.insert(L.vitems.end(), views[q].begin(), views[q].end());
+            L.max_vitem_rows = std::max<int32_t>(L.max_vitem_rows, C.m);
+        }
+    }
+    // forward packs: segments of one format (by decreasing rows, so that the
+    // lanes of a pack stream similar row counts), <= 64 groups and <= kPackXs
+    // x values per pack
     for (int f = 0; f < NFMT; ++f) {
         VPack cur{};
+        int64_t xused = 0;
         cur.fmt = f;
         auto flush = [&]() {
             if (cur.nseg > 0) L.vpacks.push_back(cur);
             cur = VPack{};
             cur.fmt = f;
+            xused = 0;
         };
-        const int W = fmt_width(f);
-        for (size_t j = 0; j < L.vitems.size(); ++j) {
-            const VItem& v = L.vitems[j];
-            if (v.fmt != f) continue;
-            if (cur.nseg == kPackSegs || cur.gofs[cur.nseg] + v.ng > kItemGroups) flush();
-            const ColPanel& C = L.cp[v.q];
-            const int s = cur.nseg;
-            cur.gofs[s + 1] = cur.gofs[s] + v.ng;
-            cur.m[s] = C.m;
-            cur.ng[s] = v.ng;
-            cur.x0[s] = int32_t(C.col0);
-            cur.vc0[s] = int32_t(C.vcol0[f] + int64_t(v.g0) * W);
-            cur.data_off[s] = v.data_off;
+        std::vector<int64_t> order;
+        for (int64_t j = 0; j < int64_t(L.fitems.size()); ++j)
+            if (L.fitems[j].fmt == f) order.push_back(j);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t a, int64_t b) { return L.fitems[a].m > L.fitems[b].m; });
+        for (int64_t oj : order) {
+            const VItem& v = L.fitems[oj];
+            if (cur.nseg == kPackSegs || cur.gofs[cur.nseg] + v.ng > kItemGroups || xused + v.m + 1 > kPackXs) flush();
+            const int j = cur.nseg;
+            cur.gofs[j + 1] = cur.gofs[j] + v.ng;
+            cur.m[j] = v.m;
+            cur.ng[j] = v.ng;
+            cur.xoff[j] = int32_t(xused);
+            cur.x0[j] = int32_t(v.x0);
+            cur.vc0[j] = v.vc0;
+            cur.data_off[j] = v.data_off;
+            xused += v.m + 1;          // odd distance between slices (LDS banks)
             ++cur.nseg;
         }
         flush();
@@ -716,32 +839,51 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         }
     }
 
-    // ---- fill column panels (V rows; item blocks row-major) ----
+    // ---- fill V segments (row-major, lane = group), parallel over column sets ----
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t ci = 0; ci < int64_t(csets.size()); ++ci) {
+        const ColSet& cs = csets[size_t(ci)];
+        for (int64_t u = 0; u < nunit[cs.b0]; ++u) {
+            int64_t cnt[NFMT] = {};
+            for (int64_t i : cs.blocks) {
+                const hmvm_block& b = blocks[i];
+                const int64_t e0 = L.blk_t0[i];
+                for (int64_t l = 0; l < b.rank; ++l) {
+                    const int f = L.tfmt[e0 + l];
+                    if (!tall(i, f)) continue;
+                    const int64_t per = int64_t(kItemGroups) * fmt_width(f);     // columns per segment
+                    const int64_t c = cnt[f]++;
+                    const VItem& sg = L.fitems[size_t(useg0[size_t(cs.unit0 + u)][f] + c / per)];
+                    const int64_t cc = c % per;
+                    const double* src = arena + b.off_b + l * int64_t(b.ncols) + (sg.x0 - b.col_ofs);
+                    const float sc = store_vals(f, sg.data_off + cc, int64_t(sg.ng) * fmt_width(f), sg.m, src);
+                    const int64_t vc = sg.vc0 + cc;
+                    L.vdest[vc] = L.pt_off[e0 + l] + int32_t(u) * int32_t(b.rank);
+                    L.vsrc[vc] = int32_t(e0 + l);
+                    if (need_scale) L.vscale[vc] = sc;
+                }
+            }
+        }
+    }
+
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t q = 0; q < NC; ++q) {
-        const ColPanel& C = L.cp[q];
-        const int64_t m = C.m;
         int64_t cnt[NFMT] = {};
-        for (const Seg& s : cseg[q]) {
-            const hmvm_block& b = blocks[s.b];
-            const int64_t r0 = C.col0 - b.col_ofs;
-            const int64_t e0 = L.blk_t0[s.b];
+        const int64_t x0 = cstart[q], m = cpan[q].second;
+        for (const Seg& sgm : cseg[q]) {
+            const hmvm_block& b = blocks[sgm.b];
+            const int64_t e0 = L.blk_t0[sgm.b];
             for (int64_t l = 0; l < b.rank; ++l) {
                 const int f = L.tfmt[e0 + l];
-                const int W = fmt_width(f);
+                if (tall(sgm.b, f)) continue;
+                const int64_t per = int64_t(kItemGroups) * fmt_width(f);
                 const int64_t c = cnt[f]++;
-                // the item holding column c of format f
-                const VItem* it = nullptr;
-                for (int64_t j = C.item0; j < C.item0 + C.nitems; ++j) {
-                    const VItem& v = L.vitems[j];
-                    if (v.fmt == f && c >= int64_t(v.g0) * W && c < int64_t(v.g0 + v.ng) * W) { it = &v; break; }
-                }
-                const int64_t ld = int64_t(it->ng) * W;
-                const int64_t o = it->data_off + (c - int64_t(it->g0) * W);
-                const double* src = arena + b.off_b + l * int64_t(b.ncols) + r0;
-                const float sc = store_vals(f, o, ld, m, src);
-                const int64_t vc = C.vcol0[f] + c;
-                L.vdest[vc] = L.pt_off[e0 + l] + s.pi * int32_t(b.rank);
+                const VItem& sg = L.fitems[size_t(qseg0[size_t(q)][f] + c / per)];
+                const int64_t cc = c % per;
+                const double* src = arena + b.off_b + l * int64_t(b.ncols) + (x0 - b.col_ofs);
+                const float sc = store_vals(f, sg.data_off + cc, int64_t(sg.ng) * fmt_width(f), m, src);
+                const int64_t vc = sg.vc0 + cc;
+                L.vdest[vc] = L.pt_off[e0 + l] + sgm.pi * int32_t(b.rank);
                 L.vsrc[vc] = int32_t(e0 + l);
                 if (need_scale) L.vscale[vc] = sc;
             }
@@ -809,7 +951,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     int64_t s8r = 0, s8v = 0;
     for (const RChunk& c : L.rchunks)
         if (fmt_scaled(c.fmt)) s8r += int64_t(c.ngroups) * fmt_width(c.fmt);
-    for (const VItem& v : L.vitems)
+    for (const VItem& v : L.fitems)
         if (fmt_scaled(v.fmt)) s8v += int64_t(v.ng) * fmt_width(v.fmt);
     L.scale_bytes = 4 * (s8r + s8v);
     br += 4.0 * double(s8r);
@@ -892,7 +1034,7 @@ int unpack_block(const HostLayout& L, int64_t bi, const void* const arenas[NFMT]
             const int f = v.fmt, W = fmt_width(f);
             const int64_t ld = int64_t(v.ng) * W;
             for (int64_t cc = 0; cc < ld; ++cc) {
-                const int64_t vc = C.vcol0[f] + int64_t(v.g0) * W + cc;
+                const int64_t vc = v.vc0 + cc;
                 const int32_t e = L.vsrc[vc];
                 if (e < e0 || e >= e0 + b.rank) continue;
                 const int64_t l = e - e0;
@@ -979,7 +1121,7 @@ int arena_source(const HostLayout& L, int f, int64_t* src) {
             if (v.fmt != f) continue;
             const int64_t ld = int64_t(v.ng) * W;
             for (int64_t cc = 0; cc < ld; ++cc) {
-                const int32_t e = L.vsrc[C.vcol0[f] + int64_t(v.g0) * W + cc];
+                const int32_t e = L.vsrc[v.vc0 + cc];
                 if (e < 0) continue;
                 const int64_t b = ent_block[size_t(e)];
                 const hmvm_block& B = L.blocks[b];

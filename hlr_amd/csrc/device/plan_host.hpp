@@ -20,7 +20,8 @@ struct HostLayout {
     std::vector<RowPanel> rp;
     std::vector<RChunk> rchunks;
     std::vector<ColPanel> cp;
-    std::vector<VItem> vitems;     // V segments, panel-major
+    std::vector<VItem> fitems;     // V segments (forward passes), by column set, unit and format
+    std::vector<VItem> vitems;     // their views per column panel, panel-major (transposed, gradient)
     std::vector<VPack> vpacks;     // forward V work units
 
     // per R column (all row panels, chunk order)
