@@ -22,15 +22,16 @@ ERRORS = {
 }
 OP_NORMAL, OP_TRANSPOSED, OP_ADJOINT, OP_CONJUGATE = 0, 1, 2, 3
 BLOCK_DENSE, BLOCK_LOWRANK, BLOCK_LOWRANK_SV = 0, 1, 2
-PREC_FP64, PREC_FP32, PREC_VALR, PREC_BFP = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_VALR, PREC_BFP, PREC_AFL = 0, 1, 2, 3, 4
 F64, F32 = 0, 1
 # size of the fixed device scratch of hmvm_apply_noalloc (HMVM_NOALLOC_MAX_BYTES)
 NOALLOC_MAX_BYTES = 64 * 1024 * 1024
 FMT_F64, FMT_F32, FMT_BF16, FMT_FP8 = 0, 1, 2, 3
 FMT_B8, FMT_B16, FMT_B25 = 4, 5, 6
-NFMT = 7
+FMT_A12, FMT_A16, FMT_A24 = 7, 8, 9
+NFMT = 10
 # bits per stored value of each format; a 16-byte row holds 128 // bits values
-FMT_BITS = (64, 32, 16, 8, 8, 16, 25)
+FMT_BITS = (64, 32, 16, 8, 8, 16, 25, 12, 16, 24)
 
 KERNEL_LOG, KERNEL_LOG_GALERKIN, KERNEL_MATERN, KERNEL_INVR = 0, 1, 2, 3
 GEOM_LINE, GEOM_GRID1D, GEOM_CUBE, GEOM_SPHERE = 0, 1, 2, 3

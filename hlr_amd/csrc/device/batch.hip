@@ -244,6 +244,7 @@ int launch(hmvm_batch* b, BatchMat* mats, hipStream_t st) {
     case kFamHw: go(std::integral_constant<int, kFamHw>()); break;
     case kFamBfp: go(std::integral_constant<int, kFamBfp>()); break;
     case kFamF64: go(std::integral_constant<int, kFamF64>()); break;
+    case kFamAfl: go(std::integral_constant<int, kFamAfl>()); break;
     default: go(std::integral_constant<int, kFamAll>()); break;
     }
     return hipGetLastError() == hipSuccess ? HMVM_OK : HMVM_ERR_DEVICE;

@@ -514,6 +514,7 @@ void launch_rtrn(const DevPlan& D, const RtItem* ids, int64_t n, const XT* x, do
 template <typename F>
 void by_family(int fam, F&& f) {
     if (fam == kFamBfp) f(std::integral_constant<int, kFamBfp>());
+    else if (fam == kFamAfl) f(std::integral_constant<int, kFamAfl>());
     else f(std::integral_constant<int, kFamHw>());
 }
 

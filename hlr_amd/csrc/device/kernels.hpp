@@ -149,6 +149,38 @@ template <int NB> struct Elem<Bfp<NB>> {
     __device__ static inline void load(bytep p, float* out) { decode(raw16(p), out); }
 };
 
+// adaptive float with NB-bit codes (tag type; the arena holds packed rows)
+template <int NB> struct Afl {};
+constexpr int afl_format(int nb) {
+    return nb == 12 ? A12 : nb == 16 ? A16 : nb == 24 ? A24 : -1;
+}
+
+template <int NB> struct Elem<Afl<NB>> {
+    static constexpr int F = afl_format(NB);
+    static_assert(F >= kFirstAfl, "no storage format with this width");
+    static constexpr int W = 128 / NB;
+    static constexpr bool kScaled = true;
+    using Acc = float;
+    static constexpr int M = NB - 1 - kAflExpBits;
+    // +-(a - 1) with a = 2^(e + 1) (1 + mant 2^-M) built from its FP32 bits
+    // (biased exponent 128 + e), the subtraction in FP32; the all-ones code
+    // is zero.  The column scale rides on z (or on the partial sums).
+    __device__ static inline void decode(u32x4 v, float* out) {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        constexpr uint32_t kZero = (1u << NB) - 1u;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const uint32_t c = afl_field<NB>(w, j);
+            const uint32_t mant = c & ((1u << M) - 1u);
+            const uint32_t e = (c >> M) & ((1u << kAflExpBits) - 1u);
+            const float a = __uint_as_float(((e | 0x80u) << 23) | (mant << (23 - M)));
+            const float d = a - 1.0f;
+            out[j] = (c == kZero) ? 0.0f : ((c >> (NB - 1)) ? -d : d);
+        }
+    }
+    __device__ static inline void load(bytep p, float* out) { decode(raw16(p), out); }
+};
+
 // start of the row holding value offset `off` (a multiple of W) in an arena
 template <typename T>
 __device__ inline bytep elem_addr(const void* arena, int64_t off) {
@@ -165,7 +197,10 @@ __device__ inline bytep elem_addr(const void* arena, int64_t off) {
     case FP8: if constexpr (FAM == kFamAll || FAM == kFamHw) { using T = uint8_t; __VA_ARGS__; } break;   \
     case B8: if constexpr (FAM == kFamAll || FAM == kFamBfp) { using T = Bfp<8>; __VA_ARGS__; } break;    \
     case B16: if constexpr (FAM == kFamAll || FAM == kFamBfp) { using T = Bfp<16>; __VA_ARGS__; } break;  \
-    default: if constexpr (FAM == kFamAll || FAM == kFamBfp) { using T = Bfp<25>; __VA_ARGS__; } break;   \
+    case B25: if constexpr (FAM == kFamAll || FAM == kFamBfp) { using T = Bfp<25>; __VA_ARGS__; } break;  \
+    case A12: if constexpr (FAM == kFamAll || FAM == kFamAfl) { using T = Afl<12>; __VA_ARGS__; } break;  \
+    case A16: if constexpr (FAM == kFamAll || FAM == kFamAfl) { using T = Afl<16>; __VA_ARGS__; } break;  \
+    default: if constexpr (FAM == kFamAll || FAM == kFamAfl) { using T = Afl<24>; __VA_ARGS__; } break;   \
     }
 
 // Runs the statements with T bound to the element type of storage format f.
@@ -177,7 +212,10 @@ __device__ inline bytep elem_addr(const void* arena, int64_t off) {
     case FP8: { using T = uint8_t; __VA_ARGS__; } break;             \
     case B8: { using T = Bfp<8>; __VA_ARGS__; } break;               \
     case B16: { using T = Bfp<16>; __VA_ARGS__; } break;             \
-    default: { using T = Bfp<25>; __VA_ARGS__; } break;              \
+    case B25: { using T = Bfp<25>; __VA_ARGS__; } break;             \
+    case A12: { using T = Afl<12>; __VA_ARGS__; } break;             \
+    case A16: { using T = Afl<16>; __VA_ARGS__; } break;             \
+    default: { using T = Afl<24>; __VA_ARGS__; } break;              \
     }
 
 __device__ inline void wave_sync() {

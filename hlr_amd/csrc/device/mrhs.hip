@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const
 int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, int64_t ldx, void* y, int64_t ldy,
                       int R, int vdt, double* ws, hipStream_t st) {
     if (R < 2 || R > kMaxRp) return 1;
-    if (fam != kFamHw && fam != kFamBfp && fam != kFamF64) return 1;
+    if (fam != kFamHw && fam != kFamBfp && fam != kFamF64 && fam != kFamAfl) return 1;
     const int64_t pad = 32;
     // workspace: partials | t | XT (ncols x R)
     double* part = ws;
@@ -98,6 +98,7 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
         constexpr int NTM = decltype(ntm)::value;
         if (fam == kFamHw) k_vfwd_mm<NTM, kFamHw><<<unsigned(D.nfitems), kThr, lds, st>>>(D, xt, R, part);
         else if (fam == kFamF64) k_vfwd_mm<NTM, kFamF64><<<unsigned(D.nfitems), kThr, lds, st>>>(D, xt, R, part);
+        else if (fam == kFamAfl) k_vfwd_mm<NTM, kFamAfl><<<unsigned(D.nfitems), kThr, lds, st>>>(D, xt, R, part);
         else k_vfwd_mm<NTM, kFamBfp><<<unsigned(D.nfitems), kThr, lds, st>>>(D, xt, R, part);
     };
     auto rpass = [&](auto ntm) {
@@ -110,6 +111,7 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
         auto by_fam = [&](auto ytag) {
             if (fam == kFamHw) go(ytag, std::integral_constant<int, kFamHw>());
             else if (fam == kFamF64) go(ytag, std::integral_constant<int, kFamF64>());
+            else if (fam == kFamAfl) go(ytag, std::integral_constant<int, kFamAfl>());
             else go(ytag, std::integral_constant<int, kFamBfp>());
         };
         if (vdt == 0) by_fam(double(0));

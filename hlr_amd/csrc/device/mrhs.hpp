@@ -10,7 +10,7 @@ namespace hmvm {
 constexpr int kMrhsMax = 64;   // right-hand sides per launch sequence
 
 // Y += alpha A X for 2 <= R <= kMrhsMax right-hand sides; vdt 0: FP64
-// vectors, 1: FP32; fam: the plan's format family (kFamHw or kFamBfp).
+// vectors, 1: FP32; fam: the plan's format family (kFamHw, kFamBfp, kFamAfl or kFamF64).
 // ws holds mrhs_ws_doubles(D, R) doubles (partials, t and the
 // transposed copy of X).  Returns 0 on success.
 int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, int64_t ldx, void* y, int64_t ldy,

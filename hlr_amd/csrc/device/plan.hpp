@@ -38,10 +38,17 @@ namespace hmvm {
 // floating point: NB-bit two's-complement integers q, W = floor(128 / NB) of
 // them per 16-byte row (value j in bits [j NB, j NB + NB) of the row read as
 // a little-endian 128-bit integer), each standing for q * 2^k with one
-// power-of-two scale 2^k per panel column (as FP8's scale).
-enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, B8 = 4, B16 = 5, B25 = 6, NFMT = 7 };
-constexpr int kFirstBfp = B8;
-constexpr int kNumBfp = NFMT - kFirstBfp;
+// power-of-two scale 2^k per panel column (as FP8's scale).  A12, A16, A24
+// are adaptive float (AFL) codes packed the same way: NB = 1 + 5 + m bits
+// (sign, 5 exponent bits, m = 6, 10, 18 mantissa bits) standing for
+// +-(a - 1) s with a = 2^(e + 1) (1 + mant 2^-m) and one scale s per panel
+// column; the all-ones code is zero.
+enum Fmt : int32_t { F64 = 0, F32 = 1, BF16 = 2, FP8 = 3, B8 = 4, B16 = 5, B25 = 6, A12 = 7, A16 = 8, A24 = 9,
+                     NFMT = 10 };
+constexpr int kFirstBfp = B8;                 // first packed format (4-word rows in host arenas)
+constexpr int kNumBfp = NFMT - kFirstBfp;     // packed formats: block floating point and AFL
+constexpr int kFirstAfl = A12;
+constexpr int kAflExpBits = 5;
 
 // Format families: a plan holds formats of one family only (the plan
 // builder enforces it), so a kernel instantiated for that family carries the
@@ -49,7 +56,8 @@ constexpr int kNumBfp = NFMT - kFirstBfp;
 // belong to both.
 // kFamF64: a plan that stores FP64 values only (the matrix-core kernels
 // have an instantiation of their own for it)
-enum FmtFamily : int { kFamAll = 0, kFamHw = 1, kFamBfp = 2, kFamF64 = 3 };
+// kFamAfl: FP64 / FP32 and the AFL formats
+enum FmtFamily : int { kFamAll = 0, kFamHw = 1, kFamBfp = 2, kFamF64 = 3, kFamAfl = 4 };
 
 constexpr int kPanel = 64;          // rows per panel = lanes per wave
 constexpr int kWaves = 4;           // waves per workgroup
@@ -59,10 +67,16 @@ constexpr int32_t kPadCol = INT32_MIN;
 
 // bits per stored value, values per 16-byte row
 HMVM_HD constexpr int fmt_bits(int f) {
-    return f == F64 ? 64 : f == F32 ? 32 : f == BF16 ? 16 : f == FP8 ? 8 : f == B8 ? 8 : f == B16 ? 16 : 25;
+    return f == F64 ? 64 : f == F32 ? 32 : f == BF16 ? 16 : f == FP8 ? 8 : f == B8 ? 8 : f == B16 ? 16
+         : f == B25 ? 25 : f == A12 ? 12 : f == A16 ? 16 : 24;
 }
 HMVM_HD constexpr int fmt_width(int f) { return 128 / fmt_bits(f); }
-HMVM_HD constexpr bool fmt_is_bfp(int f) { return f >= kFirstBfp; }
+// packed in 16-byte rows of NB-bit fields (host arenas of 32-bit words)
+HMVM_HD constexpr bool fmt_packed(int f) { return f >= kFirstBfp; }
+HMVM_HD constexpr bool fmt_is_bfp(int f) { return f >= kFirstBfp && f < kFirstAfl; }
+HMVM_HD constexpr bool fmt_is_afl(int f) { return f >= kFirstAfl && f < NFMT; }
+// mantissa bits of an AFL format
+HMVM_HD constexpr int afl_mant(int f) { return fmt_bits(f) - 1 - kAflExpBits; }
 // formats whose panel columns carry a power-of-two scale
 HMVM_HD constexpr bool fmt_scaled(int f) { return f >= FP8; }
 // bytes of `values` stored values (a multiple of the format's width)
@@ -75,6 +89,15 @@ HMVM_HD inline int32_t bfp_field(const uint32_t (&w)[4], int j) {
     uint32_t u = w[word] >> off;
     if (off + NB > 32) u |= w[word + 1] << (32 - off);
     return int32_t(u << (32 - NB)) >> (32 - NB);
+}
+
+// NB-bit field j of a 16-byte row, unsigned
+template <int NB>
+HMVM_HD inline uint32_t afl_field(const uint32_t (&w)[4], int j) {
+    const int bit = j * NB, word = bit >> 5, off = bit & 31;
+    uint32_t u = w[word] >> off;
+    if (off + NB > 32) u |= w[word + 1] << (32 - off);
+    return (NB == 32) ? u : (u & ((1u << NB) - 1u));
 }
 
 struct RowPanel {

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -102,16 +103,76 @@ int32_t bfp_quant(double v, int k, int nb) {
 
 namespace {
 
-// NB of each block floating point format, narrowest first
-constexpr int kBfpBits[kNumBfp] = {8, 16, 25};
+// NB of each block floating point format and of each AFL format, narrowest first
+constexpr int kNumWidths = 3;
+constexpr int kBfpBits[kNumWidths] = {8, 16, 25};
+constexpr int kAflBits[kNumWidths] = {12, 16, 24};
 static_assert(fmt_bits(B8) == 8 && fmt_bits(B16) == 16 && fmt_bits(B25) == 25, "format table");
+static_assert(fmt_bits(A12) == 12 && fmt_bits(A16) == 16 && fmt_bits(A24) == 24, "format table");
+static_assert(B25 - B8 + 1 == kNumWidths && A24 - A12 + 1 == kNumWidths, "format table");
 
-// sum of squared errors of one panel column stored with NB-bit values
-double bfp_err2(const double* src, int64_t m, int nb) {
+// ---- adaptive float (compress/afl.hh:220-420 with fixed code widths) ----
+// A value v of a panel column with scale s is a = |v| / s + 1 >= 2; its code
+// keeps the sign, the low kAflExpBits bits of a's binary exponent (stored as
+// exponent - 1: a's biased FP32 exponent modulo 2^5) and the leading
+// mantissa bits of a, truncated.  The all-ones code is zero.
+
+// scale of a panel column: its smallest non-zero |v| (afl.hh:835-854),
+// raised to max|v| 2^-(2^5) where the exponent field cannot span the
+// column's range, rounded down to an FP32 value
+float afl_scale(const double* v, int64_t m) {
+    double vmax = 0, vmin = HUGE_VAL;
+    for (int64_t i = 0; i < m; ++i) {
+        const double a = std::fabs(v[i]);
+        vmax = std::max(vmax, a);
+        if (a > 0) vmin = std::min(vmin, a);
+    }
+    if (!(vmax > 0)) return 1.0f;
+    const double s = std::max(vmin, std::ldexp(vmax, -(1 << kAflExpBits)));
+    float f = float(s);
+    if (double(f) > s) f = std::nextafter(f, 0.0f);
+    return f > 0 ? f : FLT_MIN;
+}
+
+uint32_t afl_code(double v, float s, int nb) {
+    const int m = nb - 1 - kAflExpBits;
+    const uint32_t zero = (1u << nb) - 1u;
+    const double a = std::fabs(v) / double(s) + 1.0;
+    if (!(a >= 2.0) || !std::isfinite(a)) return zero;     // below the scale: zero
+    int E;
+    const double fr = std::frexp(a, &E);                   // a = fr 2^E, fr in [0.5, 1)
+    if (E - 2 > (1 << kAflExpBits) - 1) return zero;       // beyond the range (not for a column's own scale)
+    const uint32_t mant = uint32_t(std::floor(std::ldexp(2.0 * fr - 1.0, m)));
+    return (uint32_t(v < 0) << (nb - 1)) | (uint32_t(E - 2) << m) | mant;
+}
+
+// +-(a - 1) of a code, in FP32 as the kernels decode it (scale not applied)
+float afl_unscaled(uint32_t c, int nb) {
+    const int m = nb - 1 - kAflExpBits;
+    if (c == (1u << nb) - 1u) return 0.0f;
+    const uint32_t e = (c >> m) & ((1u << kAflExpBits) - 1u), mant = c & ((1u << m) - 1u);
+    const uint32_t bits = ((e | 0x80u) << 23) | (mant << (23 - m));
+    float a;
+    std::memcpy(&a, &bits, 4);
+    const float d = a - 1.0f;
+    return (c >> (nb - 1)) ? -d : d;
+}
+
+// sum of squared errors of one panel column stored with NB-bit values of
+// the block floating point (afl false) or AFL (afl true) formats
+double packed_err2(const double* src, int64_t m, int nb, bool afl) {
+    double e2 = 0;
+    if (afl) {
+        const float s = afl_scale(src, m);
+        for (int64_t i = 0; i < m; ++i) {
+            const double d = double(afl_unscaled(afl_code(src[i], s, nb), nb)) * double(s) - src[i];
+            e2 += d * d;
+        }
+        return e2;
+    }
     double mx = 0;
     for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
     const int k = bfp_exp(mx, nb);
-    double e2 = 0;
     for (int64_t i = 0; i < m; ++i) {
         const double d = std::ldexp(double(bfp_quant(src[i], k, nb)), k) - src[i];
         e2 += d * d;
@@ -124,11 +185,11 @@ struct Piece {            // one panel column: m values from src
     int64_t m;
 };
 
-// the narrowest block floating point format whose stored pieces of every
-// column in `cols` are within tol (relative, 2-norm) of the column; else
-// FP32 when rate <= 23 (its rounding error 2^-24 <= 2^-(rate+1) <= tol), else
-// FP64
-int bfp_choose(const std::vector<std::vector<Piece>>& cols, double tol, int rate) {
+// the narrowest packed format (block floating point, or AFL when afl) whose
+// stored pieces of every column in `cols` are within tol (relative, 2-norm)
+// of the column; else FP32 when rate <= 23 (its rounding error
+// 2^-24 <= 2^-(rate+1) <= tol), else FP64
+int packed_choose(const std::vector<std::vector<Piece>>& cols, double tol, int rate, bool afl) {
     std::vector<double> thr(cols.size());
     for (size_t c = 0; c < cols.size(); ++c) {
         double n2 = 0;
@@ -136,21 +197,22 @@ int bfp_choose(const std::vector<std::vector<Piece>>& cols, double tol, int rate
             for (int64_t i = 0; i < pc.m; ++i) n2 += pc.src[i] * pc.src[i];
         thr[c] = (tol * tol) * n2;
     }
-    for (int f = 0; f < kNumBfp; ++f) {
+    for (int f = 0; f < kNumWidths; ++f) {
+        const int nb = afl ? kAflBits[f] : kBfpBits[f];
         bool ok = true;
         for (size_t c = 0; c < cols.size() && ok; ++c) {
             double e2 = 0;
-            for (const Piece& pc : cols[c]) e2 += bfp_err2(pc.src, pc.m, kBfpBits[f]);
+            for (const Piece& pc : cols[c]) e2 += packed_err2(pc.src, pc.m, nb, afl);
             ok = (e2 <= thr[c]);
         }
-        if (ok) return kFirstBfp + f;
+        if (ok) return (afl ? kFirstAfl : kFirstBfp) + f;
     }
     return rate <= 23 ? F32 : F64;
 }
 
-// write NB-bit value q as field j of the 16-byte row w[0..3]
-void bfp_put(uint32_t* w, int j, int nb, int32_t q) {
-    const uint32_t u = uint32_t(q) & ((1u << nb) - 1u);
+// write NB-bit field u (the low NB bits are kept) as field j of the 16-byte row w[0..3]
+void packed_put(uint32_t* w, int j, int nb, uint32_t u) {
+    u &= (1u << nb) - 1u;
     const int bit = j * nb, word = bit >> 5, off = bit & 31;
     w[word] |= u << off;
     if (off + nb > 32) w[word + 1] |= u >> (32 - off);
@@ -219,8 +281,10 @@ struct Seg {              // a block's contribution to one panel
 int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t ncols, const double* arena,
                  int64_t arena_len, const hmvm_plan_opts& opts, HostLayout& L) {
     if (nb < 0 || nrows < 0 || ncols < 0 || (nb > 0 && (!blocks || !arena))) return HMVM_ERR_INVALID_ARG;
-    if (opts.precision < HMVM_PREC_FP64 || opts.precision > HMVM_PREC_BFP) return HMVM_ERR_INVALID_ARG;
-    const bool adaptive = (opts.precision == HMVM_PREC_VALR || opts.precision == HMVM_PREC_BFP);
+    if (opts.precision < HMVM_PREC_FP64 || opts.precision > HMVM_PREC_AFL) return HMVM_ERR_INVALID_ARG;
+    const bool packed = (opts.precision == HMVM_PREC_BFP || opts.precision == HMVM_PREC_AFL);
+    const bool afl = (opts.precision == HMVM_PREC_AFL);
+    const bool adaptive = (opts.precision == HMVM_PREC_VALR || packed);
     if (adaptive && !(opts.eps > 0)) return HMVM_ERR_INVALID_ARG;
     if (opts.force_format < 0 || opts.force_format > NFMT) return HMVM_ERR_INVALID_ARG;
     if (nrows > INT32_MAX || ncols > INT32_MAX) return HMVM_ERR_UNSUPPORTED;
@@ -231,8 +295,9 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     L.eps = opts.eps;
     // dense leaves under VALR: as requested, or (dense_prec == VALR) by the
     // same rate rule as the columns, restricted to FP32 / FP64: FP32 while
-    // eps needs at most its 23 mantissa bits.  Under BFP with dense_prec ==
-    // VALR every dense column gets the block floating point rule at rate(eps).
+    // eps needs at most its 23 mantissa bits.  Under BFP / AFL with
+    // dense_prec == VALR every dense column gets the plan's packed-format
+    // rule at rate(eps).
     if (adaptive && opts.dense_prec != HMVM_PREC_FP64 &&
         opts.dense_prec != HMVM_PREC_FP32 && opts.dense_prec != HMVM_PREC_VALR)
         return HMVM_ERR_INVALID_ARG;
@@ -241,7 +306,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 : (opts.dense_prec == HMVM_PREC_FP64) ? F64
                 : (opts.dense_prec == HMVM_PREC_FP32) ? F32
                 : (eps_to_rate(opts.eps) <= 23 ? F32 : F64);
-    const bool dense_bfp = (opts.precision == HMVM_PREC_BFP && opts.dense_prec == HMVM_PREC_VALR);
+    const bool dense_packed = (packed && opts.dense_prec == HMVM_PREC_VALR);
     L.blocks.assign(blocks, blocks + nb);
 
     // ---- validate leaves (cf. HLR_ASSERT in apply_add, check_data()) ----
@@ -368,8 +433,9 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 for (int64_t l = 0; l < b.rank; ++l) L.tfmt[e0 + l] = int8_t(opts.force_format - 1);
         }
     }
-    // block floating point: U and V column l of a block share one width, the
-    // narrowest whose panel columns meet the 2-norm bound for rate(u_l)
+    // block floating point / AFL: U and V column l of a block share one
+    // width, the narrowest whose panel columns meet the 2-norm bound for
+    // the column's VALR tolerance
     auto upieces = [&](int64_t i, int64_t l) {
         const hmvm_block& b = blocks[i];
         std::vector<Piece> v;
@@ -377,7 +443,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
             v.push_back({arena + b.off_a + l * int64_t(b.nrows) + (rstart[p] - b.row_ofs), rpan[p].second});
         return v;
     };
-    if (opts.precision == HMVM_PREC_BFP && opts.force_format == 0) {
+    if (packed && opts.force_format == 0) {
 #pragma omp parallel for schedule(dynamic, 16)
         for (int64_t i = 0; i < nb; ++i) {
             const hmvm_block& b = blocks[i];
@@ -390,7 +456,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 for (int64_t q = bq0[i]; q < bq1[i]; ++q)
                     cols[1].push_back({arena + b.off_b + l * int64_t(b.ncols) + (cstart[q] - b.col_ofs),
                                        cpan[q].second});
-                L.tfmt[L.blk_t0[i] + l] = int8_t(bfp_choose(cols, tol[l], eps_to_rate(tol[l])));
+                L.tfmt[L.blk_t0[i] + l] = int8_t(packed_choose(cols, tol[l], eps_to_rate(tol[l]), afl));
             }
         }
     }
@@ -402,7 +468,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         for (int64_t i = 0; i < nb; ++i)
             if (blocks[i].kind == HMVM_BLOCK_DENSE) { L.blk_d0[i] = nd; nd += blocks[i].ncols; }
         L.dfmt.assign(nd, int8_t(L.dense_fmt));
-        if (dense_bfp) {
+        if (dense_packed) {
             // a dense column has no VALR tolerance of its own: eps's rate plus
             // one bit, the bound of a rounding format with rate(eps) bits
             const int rate = eps_to_rate(opts.eps);
@@ -412,7 +478,7 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
                 if (blocks[i].kind != HMVM_BLOCK_DENSE) continue;
                 for (int64_t j = 0; j < blocks[i].ncols; ++j) {
                     std::vector<std::vector<Piece>> cols(1, upieces(i, j));
-                    L.dfmt[L.blk_d0[i] + j] = int8_t(bfp_choose(cols, dtol, rate));
+                    L.dfmt[L.blk_d0[i] + j] = int8_t(packed_choose(cols, dtol, rate, afl));
                 }
             }
         }
@@ -778,13 +844,21 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
         }
         default: {
             const int nb = fmt_bits(f), W = fmt_width(f);
+            uint32_t* words = L.abfp[f - kFirstBfp].data();
+            if (fmt_is_afl(f)) {
+                const float s = afl_scale(src, m);
+                for (int64_t i = 0; i < m; ++i) {
+                    const int64_t v = o + i * ld;
+                    packed_put(words + (v / W) * 4, int(v % W), nb, afl_code(src[i], s, nb));
+                }
+                return s;
+            }
             double mx = 0;
             for (int64_t i = 0; i < m; ++i) mx = std::max(mx, std::fabs(src[i]));
             const int k = bfp_exp(mx, nb);
-            uint32_t* words = L.abfp[f - kFirstBfp].data();
             for (int64_t i = 0; i < m; ++i) {
                 const int64_t v = o + i * ld;
-                bfp_put(words + (v / W) * 4, int(v % W), nb, bfp_quant(src[i], k, nb));
+                packed_put(words + (v / W) * 4, int(v % W), nb, uint32_t(bfp_quant(src[i], k, nb)));
             }
             return std::ldexp(1.0f, k);
         }
@@ -911,14 +985,16 @@ int build_layout(const hmvm_block* blocks, int64_t nb, int64_t nrows, int64_t nc
     // ---- statistics ----
     for (int f = 0; f < NFMT; ++f) L.bytes_by_format[f] = fmt_store_bytes(f, L.nvals[f]);
     {
-        bool hw = false, bfp = false;
+        bool hw = false, bfp = false, aflf = false;
         for (int f = 0; f < NFMT; ++f) {
             if (L.nvals[f] == 0) continue;
             hw |= (f == BF16 || f == FP8);
             bfp |= fmt_is_bfp(f);
+            aflf |= fmt_is_afl(f);
         }
-        if (hw && bfp) return HMVM_ERR_INVALID_ARG;     // e.g. BF16 forced beside block floating point dense leaves
-        L.family = bfp ? kFamBfp : kFamHw;
+        // e.g. BF16 forced beside block floating point dense leaves
+        if (int(hw) + int(bfp) + int(aflf) > 1) return HMVM_ERR_INVALID_ARG;
+        L.family = bfp ? kFamBfp : aflf ? kFamAfl : kFamHw;
     }
     // bytes per stored value (16 bytes per row of W values)
     auto vbytes = [](int f) { return 16.0 / double(fmt_width(f)); };
@@ -976,6 +1052,14 @@ static int32_t bfp_get(const uint32_t* words, int64_t v) {
     return bfp_field<NB>(w, int(v % W));
 }
 
+template <int NB>
+static double afl_get(const uint32_t* words, int64_t v) {
+    constexpr int W = 128 / NB;
+    const uint32_t* r = words + (v / W) * 4;
+    const uint32_t w[4] = {r[0], r[1], r[2], r[3]};
+    return double(afl_unscaled(afl_field<NB>(w, int(v % W)), NB));
+}
+
 // stored value idx of format f (without its panel-column scale)
 static double load_elem(int f, const void* const arenas[NFMT], int64_t idx, bool as_f64) {
     if (as_f64) return static_cast<const double*>(arenas[f])[idx];
@@ -987,7 +1071,10 @@ static double load_elem(int f, const void* const arenas[NFMT], int64_t idx, bool
     case FP8: return decode_e4m3(static_cast<const uint8_t*>(arenas[FP8])[idx]);
     case B8: return bfp_get<8>(w, idx);
     case B16: return bfp_get<16>(w, idx);
-    default: return bfp_get<25>(w, idx);
+    case B25: return bfp_get<25>(w, idx);
+    case A12: return afl_get<12>(w, idx);
+    case A16: return afl_get<16>(w, idx);
+    default: return afl_get<24>(w, idx);
     }
 }
 

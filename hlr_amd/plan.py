@@ -13,8 +13,8 @@ import numpy as np
 
 from . import _native as N
 
-PRECISIONS = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "valr": N.PREC_VALR, "bfp": N.PREC_BFP}
-FORMAT_NAMES = ("fp64", "fp32", "bf16", "fp8_e4m3", "bfp8", "bfp16", "bfp25")
+PRECISIONS = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "valr": N.PREC_VALR, "bfp": N.PREC_BFP, "afl": N.PREC_AFL}
+FORMAT_NAMES = ("fp64", "fp32", "bf16", "fp8_e4m3", "bfp8", "bfp16", "bfp25", "afl12", "afl16", "afl24")
 # bytes per stored value (16 bytes hold 128 // bits values)
 FORMAT_BYTES = tuple(16.0 / (128 // b) for b in N.FMT_BITS)
 
@@ -103,9 +103,9 @@ class Plan:
         arena = np.ascontiguousarray(arena, dtype=np.float64)
         opts = N.HmvmPlanOpts()
         opts.precision = PRECISIONS[precision]
-        # dense leaves under "valr" / "bfp": "fp64", "fp32", or "auto" (valr:
-        # FP32 while eps needs at most 23 mantissa bits, else FP64; bfp: the
-        # block floating point column rule at rate(eps))
+        # dense leaves under "valr" / "bfp" / "afl": "fp64", "fp32", or "auto"
+        # (valr: FP32 while eps needs at most 23 mantissa bits, else FP64;
+        # bfp / afl: the plan's packed-format column rule at rate(eps))
         opts.dense_prec = {"fp64": N.PREC_FP64, "fp32": N.PREC_FP32, "auto": N.PREC_VALR}[dense_prec]
         opts.eps = float(eps)
         opts.check_finite = 1 if check_finite else 0

@@ -69,12 +69,18 @@ enum {
     HMVM_PREC_VALR = 2,   /* per-column FP64/FP32/BF16/FP8-E4M3 chosen from singular
                              values against eps (compress/valr.hh:402-512,
                              compress/afl.hh:81-82,854-856); dense leaves FP32          */
-    HMVM_PREC_BFP  = 3    /* per-column block floating point (compress/bfp.hh:765-860
+    HMVM_PREC_BFP  = 3,   /* per-column block floating point (compress/bfp.hh:765-860
                              with VALR tolerances): the narrowest of the B8, B16,
                              B25 formats whose stored column, U and V alike, is within
                              the column's VALR tolerance of the FP64 column in the
                              2-norm; FP32 / FP64 when none is.  Dense leaves per
                              column within 2^-(rate(eps)+1)                          */
+    HMVM_PREC_AFL  = 4    /* per-column adaptive float (compress/afl.hh:220-420,
+                             compress_lr :818-880 with VALR tolerances): the narrowest
+                             of the A12, A16, A24 formats whose stored column, U
+                             and V alike, is within the column's VALR tolerance in
+                             the 2-norm; FP32 / FP64 when none is.  Dense leaves as
+                             under HMVM_PREC_BFP, with the AFL formats              */
 };
 
 /* vector element type at the boundary */
@@ -113,23 +119,30 @@ enum {
        power-of-two step 2^k per panel column                                 */
     HMVM_FMT_B8   = 4,    /* NB =  8, 16 values per row */
     HMVM_FMT_B16  = 5,    /* NB = 16,  8 values per row */
-    HMVM_FMT_B25  = 6     /* NB = 25,  5 values per row */
+    HMVM_FMT_B25  = 6,    /* NB = 25,  5 values per row */
+    /* adaptive float (AFL): NB-bit codes packed as the block floating point
+       values; code = (sign, 5 exponent bits e, m mantissa bits) from the
+       high bit down, standing for +-(a - 1) s with a = 2^(e+1) (1 + mant 2^-m)
+       evaluated in FP32, s an FP32 scale per panel column; all-ones = 0    */
+    HMVM_FMT_A12  = 7,    /* NB = 12, m =  6, 10 values per row */
+    HMVM_FMT_A16  = 8,    /* NB = 16, m = 10,  8 values per row */
+    HMVM_FMT_A24  = 9     /* NB = 24, m = 18,  5 values per row */
 };
-#define HMVM_NFMT 7
+#define HMVM_NFMT 10
 
 typedef struct hmvm_plan_opts {
     int32_t precision;    /* HMVM_PREC_*                                          */
-    int32_t dense_prec;   /* dense leaves under HMVM_PREC_VALR / _BFP: HMVM_PREC_FP64,
-                             HMVM_PREC_FP32, or HMVM_PREC_VALR = by eps (VALR: FP32
-                             while ceil(-log2 eps) <= 23, else FP64; BFP: the
-                             column rule of HMVM_PREC_BFP at rate(eps))          */
-    double  eps;          /* compression tolerance for HMVM_PREC_VALR / _BFP      */
+    int32_t dense_prec;   /* dense leaves under HMVM_PREC_VALR / _BFP / _AFL:
+                             HMVM_PREC_FP64, HMVM_PREC_FP32, or HMVM_PREC_VALR =
+                             by eps (VALR: FP32 while ceil(-log2 eps) <= 23, else
+                             FP64; BFP / AFL: the plan's column rule at rate(eps)) */
+    double  eps;          /* compression tolerance for HMVM_PREC_VALR / _BFP / _AFL */
     int32_t check_finite; /* reject NaN/Inf factors at creation (default 1)       */
     int32_t force_format; /* 0: formats from `precision`; 1 + HMVM_FMT_*: store
                              every low-rank column in that format (experiments
-                             and format tests).  A plan may not hold BF16 or
-                             FP8 columns beside block floating point ones
-                             (HMVM_ERR_INVALID_ARG)                               */
+                             and format tests).  A plan may not hold columns of
+                             two of the families BF16 / FP8, block floating
+                             point and AFL (HMVM_ERR_INVALID_ARG)                 */
 } hmvm_plan_opts;
 
 typedef struct hmvm_plan_info {
@@ -250,7 +263,7 @@ int hmvm_tlr_apply(const hmvm_plan* plan, double alpha, int op,
  * pass NULL to skip.  Each buffer must be 16-byte aligned and gx aligned to
  * its element size (HMVM_ERR_LAYOUT otherwise).  For FP64, FP32 and BF16
  * storage an entry is the gradient with respect to the stored value; for
- * the scaled formats (FP8, block floating point) it is the gradient with
+ * the scaled formats (FP8, block floating point, AFL) it is the gradient with
  * respect to the dequantised value (stored value times its column scale).
  * No counterpart in the reference (it has no differentiation); the transposed
  * product follows arith/blas.hh:910-948.

@@ -15,6 +15,11 @@ header and exercised by tests/test_oracle.py):
                                            the OCP E4M3 code table)
   bfp_exp, bfp_round, bfp_format           pinned (numpy rint, brute-force
                                            minimality, representable columns)
+  afl_scale, afl_encode, afl_decode        pinned (exactly representable grid
+                                           values, truncation onto the grid by
+                                           brute force, scale rule, zero code)
+  afl_format                               pinned (brute-force minimality over
+                                           the pinned rounding, fallbacks)
   uniform_mul_vec, uniform_dense_expand    pinned (numpy products of a
                                            hand-assembled expansion; H^2 equal
                                            to the uniform-H matrix with the
@@ -73,6 +78,16 @@ def lib() -> C.CDLL:
         L.oracle_bfp_round.restype = d
         L.oracle_bfp_format.argtypes = [i32, vp, vp, vp, d, i32]
         L.oracle_bfp_format.restype = i32
+        L.oracle_afl_scale.argtypes = [vp, i64]
+        L.oracle_afl_scale.restype = C.c_float
+        L.oracle_afl_encode.argtypes = [d, C.c_float, i32]
+        L.oracle_afl_encode.restype = C.c_uint32
+        L.oracle_afl_decode.argtypes = [C.c_uint32, C.c_float, i32]
+        L.oracle_afl_decode.restype = d
+        L.oracle_afl_round.argtypes = [d, C.c_float, i32]
+        L.oracle_afl_round.restype = d
+        L.oracle_afl_format.argtypes = [i32, vp, vp, vp, d, i32]
+        L.oracle_afl_format.restype = i32
         L.oracle_uniform_mul_vec.argtypes = [vp, i64, vp, i64, vp, i64, vp, d, i32, vp, vp]
         L.oracle_uniform_mul_vec.restype = i32
         L.oracle_uniform_dense_expand.argtypes = [vp, i64, vp, i64, vp, i64, vp, i64, i64, vp]
@@ -226,3 +241,46 @@ def uniform_dense_expand(rowcb, colcb, blocks, arena, nrows, ncols):
                                            _p(arena), nrows, ncols, _p(A))
     assert rc == 0, rc
     return A.T
+
+
+def afl_scale(col):
+    col = np.ascontiguousarray(col, dtype=np.float64)
+    return float(lib().oracle_afl_scale(_p(col), col.size))
+
+
+def afl_encode(v, scale, nb):
+    return int(lib().oracle_afl_encode(float(v), float(scale), int(nb)))
+
+
+def afl_decode(code, scale, nb):
+    return float(lib().oracle_afl_decode(int(code), float(scale), int(nb)))
+
+
+def afl_round_column(col, nb, piece=64):
+    """Stored values of a column in AFL codes of nb bits, one scale per
+    piece of `piece` rows (or the given list of piece lengths)."""
+    col = np.asarray(col, dtype=np.float64)
+    out = np.empty_like(col)
+    r0 = 0
+    for ln in _pieces(len(col), piece):
+        seg = col[r0:r0 + ln]
+        s = afl_scale(seg)
+        out[r0:r0 + ln] = [lib().oracle_afl_round(float(v), s, int(nb)) for v in seg]
+        r0 += ln
+    return out
+
+
+def afl_format(cols, tol, rate, piece=64):
+    """Format chosen for the columns `cols` sharing one AFL width under
+    relative 2-norm tolerance tol (fallback by rate); pieces as bfp_format."""
+    vals, npieces, plen = [], [], []
+    for c, col in enumerate(cols):
+        col = np.asarray(col, dtype=np.float64)
+        lens = _pieces(len(col), piece if np.isscalar(piece) else piece[c])
+        vals.append(col)
+        npieces.append(len(lens))
+        plen.extend(lens)
+    v = np.ascontiguousarray(np.concatenate(vals) if vals else np.zeros(0))
+    npc = np.ascontiguousarray(npieces, dtype=np.int64)
+    pl = np.ascontiguousarray(plen, dtype=np.int64)
+    return lib().oracle_afl_format(len(cols), _p(v), _p(npc), _p(pl), float(tol), int(rate))

@@ -54,6 +54,7 @@
  * multiplied out in numpy), and H^2 against the equal uniform-H matrix
  * whose bases are the expanded nested ones.
  */
+#include <float.h>
 #include <math.h>
 #include <omp.h>
 #include <stdint.h>
@@ -671,4 +672,110 @@ int oracle_uniform_dense_expand(const hmvm_cluster* rowcb, int64_t nr, const hmv
         free(V);
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* adaptive float (AFL) columns                                             */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * The AFL codec of compress/afl.hh:220-332 (compress) and :333-420
+ * (decompress), restated for fixed code widths: a value v of a column
+ * with scale s is stored as a = |v| / s + 1 >= 2, keeping the sign, the low
+ * ORACLE_AFL_EBITS bits of a's binary exponent (a's exponent minus one, as
+ * the reference keeps the low bits of the biased exponent above 2^7) and
+ * the leading m bits of a's mantissa, truncated; code = (sign, exponent,
+ * mantissa) from the high bits down in an NB = 1 + ebits + m bit field; the
+ * all-ones code is zero.  Decoding rebuilds a in FP32 and returns
+ * sign * fl32(a - 1) * s, the reference's ( fival.f - 1 ) * scale.
+ * The scale is the column's smallest non-zero |v| (afl.hh:835-854), raised
+ * to max|v| / 2^(2^ebits) when the exponent bits cannot span the column's
+ * range; values below the scale are stored as zero.
+ */
+#define ORACLE_AFL_EBITS 5
+
+float oracle_afl_scale(const double* v, int64_t n) {
+    double vmax = 0.0, vmin = INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        const double a = fabs(v[i]);
+        if (a > vmax) vmax = a;
+        if (a > 0.0 && a < vmin) vmin = a;
+    }
+    if (!(vmax > 0.0)) return 1.0f;
+    double s = vmin;
+    const double lo = ldexp(vmax, -(1 << ORACLE_AFL_EBITS));
+    if (s < lo) s = lo;
+    float f = (float)s;
+    if ((double)f > s) f = nextafterf(f, 0.0f);       /* not above the exact scale */
+    if (!(f > 0.0f)) f = FLT_MIN;
+    return f;
+}
+
+uint32_t oracle_afl_encode(double v, float scale, int nb) {
+    const int m = nb - 1 - ORACLE_AFL_EBITS;
+    const uint32_t zero = (nb >= 32) ? 0xffffffffu : ((1u << nb) - 1u);
+    const double a = fabs(v) / (double)scale + 1.0;
+    if (!(a >= 2.0) || !isfinite(a)) return zero;
+    int E;
+    const double fr = frexp(a, &E);                   /* a = fr 2^E, fr in [0.5, 1) */
+    const int ex = E - 1;                             /* a in [2^ex, 2^(ex+1)), ex >= 1 */
+    if (ex - 1 > (1 << ORACLE_AFL_EBITS) - 1) return zero;
+    const double frac = fr * 2.0 - 1.0;               /* mantissa in [0, 1) */
+    const uint32_t zm = (uint32_t)floor(ldexp(frac, m));
+    const uint32_t sign = (v < 0.0) ? 1u : 0u;
+    return (sign << (nb - 1)) | ((uint32_t)(ex - 1) << m) | zm;
+}
+
+double oracle_afl_decode(uint32_t code, float scale, int nb) {
+    const int m = nb - 1 - ORACLE_AFL_EBITS;
+    const uint32_t zero = (nb >= 32) ? 0xffffffffu : ((1u << nb) - 1u);
+    if (code == zero) return 0.0;
+    const uint32_t mant = code & ((1u << m) - 1u);
+    const uint32_t ex = (code >> m) & ((1u << ORACLE_AFL_EBITS) - 1u);
+    const uint32_t sign = code >> (nb - 1);
+    const uint32_t bits = ((ex | 0x80u) << 23) | (mant << (23 - m));
+    float a;
+    memcpy(&a, &bits, sizeof(a));
+    const float d = a - 1.0f;                         /* FP32, as the reference */
+    return (sign ? -1.0 : 1.0) * (double)d * (double)scale;
+}
+
+/* stored value of v: decode(encode(v)) */
+double oracle_afl_round(double v, float scale, int nb) {
+    return oracle_afl_decode(oracle_afl_encode(v, scale, nb), scale, nb);
+}
+
+/*
+ * Format of the columns `vals` (ncols columns of npieces[c] pieces of
+ * plen[..] values each, concatenated) sharing one AFL width: the narrowest
+ * of 12, 16, 24 bits whose stored pieces (scale per piece) keep every column
+ * within tol of itself in the 2-norm, relative; else FP32 while rate <= 23,
+ * else FP64.  The widths follow the per-column mantissa choice of
+ * compress/afl.hh:818-880 (compress_lr) with VALR tolerances, rounded up
+ * to three code widths.
+ */
+static const int oracle_afl_bits[3] = {12, 16, 24};
+
+int oracle_afl_format(int ncols, const double* vals, const int64_t* npieces, const int64_t* plen, double tol,
+                      int rate) {
+    for (int f = 0; f < 3; ++f) {
+        const int nb = oracle_afl_bits[f];
+        int ok = 1;
+        const double* v = vals;
+        const int64_t* pl = plen;
+        for (int c = 0; c < ncols; ++c) {
+            double n2 = 0.0, e2 = 0.0;
+            for (int64_t p = 0; p < npieces[c]; ++p, ++pl) {
+                const float s = oracle_afl_scale(v, *pl);
+                for (int64_t i = 0; i < *pl; ++i, ++v) {
+                    const double d = oracle_afl_round(v[0], s, nb) - v[0];
+                    n2 += v[0] * v[0];
+                    e2 += d * d;
+                }
+            }
+            if (!(e2 <= (tol * tol) * n2)) ok = 0;
+        }
+        if (ok) return HMVM_FMT_A12 + f;
+    }
+    return rate <= 23 ? HMVM_FMT_F32 : HMVM_FMT_F64;
 }

@@ -1,10 +1,12 @@
-"""Block floating point storage (GPU).
+"""Packed storage formats on the GPU: block floating point (B8, B16, B25)
+and adaptive float (A12, A16, A24).
 
-Bars: the product of a block floating point plan is within 2e-6 of the
-oracle run on the plan's own dequantised factors (FP32 accumulation inside a
-panel), for every width forced on all low-rank columns, forward and
-transposed, for several right-hand sides, and for the adjoint; the plan
-chosen against eps is within eps of the FP64 oracle on the same H-matrix.
+Bars: the product of a packed plan is within 2e-6 of the oracle run on the
+plan's own dequantised factors (FP32 accumulation inside a panel), for every
+width forced on all low-rank columns, forward and transposed, for several
+right-hand sides, and for the adjoint; the plan chosen against eps is within
+eps of the FP64 oracle on the same H-matrix (AFL: within eps where its
+widest code, 18 mantissa bits, and FP32 / FP64 carry the columns).
 """
 from __future__ import annotations
 
@@ -20,7 +22,9 @@ from tests.helpers import dequantized_arena, rel
 pytestmark = pytest.mark.gpu
 
 TOL_FP32ACC = 2e-6
-BFP = FORMAT_NAMES[N.FMT_B8:]
+FAMILY = {"bfp": (N.FMT_B8, N.FMT_B25), "afl": (N.FMT_A12, N.FMT_A24)}
+WIDTHS = [("bfp", f) for f in FORMAT_NAMES[N.FMT_B8:N.FMT_B25 + 1]] + \
+         [("afl", f) for f in FORMAT_NAMES[N.FMT_A12:N.FMT_A24 + 1]]
 
 
 @pytest.fixture(scope="module")
@@ -32,20 +36,20 @@ def _apply(plan, alpha, op, x, y0, cuda):
     import torch
     xt = torch.as_tensor(x, device=cuda)
     yt = torch.as_tensor(y0, device=cuda).clone()
-    print(f"[gpu] bfp apply op={op} shape={x.shape}", flush=True)
+    print(f"[gpu] packed apply op={op} shape={x.shape}", flush=True)
     plan.apply(alpha, op, xt, yt)
     torch.cuda.synchronize()
     return yt.cpu().numpy()
 
 
-@pytest.mark.parametrize("force", BFP)
+@pytest.mark.parametrize("prec,force", WIDTHS)
 @pytest.mark.parametrize("op", [N.OP_NORMAL, N.OP_TRANSPOSED])
-def test_forced_width_vs_dequantized(cuda, sphere16k, force, op):
+def test_forced_width_vs_dequantized(cuda, sphere16k, prec, force, op):
     """Every low-rank column in one width: the width's decode carries most
-    of the result, so a field offset or sign extension that differed from the
-    host packing would fail here."""
+    of the result, so a field offset, sign or exponent handling that differed
+    from the host packing would fail here."""
     hm = sphere16k
-    plan = Plan.from_hmatrix(hm, precision="bfp", eps=1e-6, force_format=force)
+    plan = Plan.from_hmatrix(hm, precision=prec, eps=1e-6, force_format=force)
     c = plan.info()["cols_by_format"]
     assert c[FORMAT_NAMES.index(force)] == sum(c)
     deq = dequantized_arena(plan, hm)
@@ -57,14 +61,16 @@ def test_forced_width_vs_dequantized(cuda, sphere16k, force, op):
     assert rel(y - y0, yr - y0) <= TOL_FP32ACC
 
 
+@pytest.mark.parametrize("prec", ["bfp", "afl"])
 @pytest.mark.parametrize("eps", [1e-4, 1e-6, 1e-8])
-def test_error_vs_fp64(cuda, eps):
+def test_error_vs_fp64(cuda, prec, eps):
     """Widths chosen against eps keep the product within eps of the FP64
     oracle (forward and transposed)."""
     hm = HMatrix(INVR_SPHERE.with_(n=16384, eps=eps))
-    plan = Plan.from_hmatrix(hm, precision="bfp", eps=eps)
+    plan = Plan.from_hmatrix(hm, precision=prec, eps=eps)
     c = plan.info()["cols_by_format"]
-    assert sum(c[N.FMT_B8:]) > 0.5 * sum(c)
+    first, last = FAMILY[prec]
+    assert sum(c[first:last + 1]) > 0.5 * sum(c)
     x = random_vector(hm.n, 23)
     deq = dequantized_arena(plan, hm)
     for op in (N.OP_NORMAL, N.OP_TRANSPOSED):
@@ -77,22 +83,24 @@ def test_error_vs_fp64(cuda, eps):
         assert rel(y, yq) <= TOL_FP32ACC
 
 
+@pytest.mark.parametrize("prec", ["bfp", "afl"])
 @pytest.mark.parametrize("nrhs", [2, 16, 33, 64])
-def test_multi_rhs(cuda, sphere16k, nrhs):
-    """Several right-hand sides on the matrix cores with all six widths in
-    one plan (the V pass splits a width into accumulator sets of 3, 4 or 5
+def test_multi_rhs(cuda, sphere16k, prec, nrhs):
+    """Several right-hand sides on the matrix cores with all of the family's
+    widths in one plan (the V pass splits a width into accumulator sets of
     columns)."""
     import torch
     hm = sphere16k
-    plan = Plan.from_hmatrix(hm, precision="bfp", eps=1e-6)
+    plan = Plan.from_hmatrix(hm, precision=prec, eps=1e-6)
     c = plan.info()["cols_by_format"]
-    assert all(c[f] > 0 for f in range(N.FMT_B8, N.NFMT)), c
+    first, last = FAMILY[prec]
+    assert all(c[f] > 0 for f in range(first, last + 1)), c
     deq = dequantized_arena(plan, hm)
     X = np.stack([random_vector(hm.n, 300 + r) for r in range(nrhs)])
     Y0 = np.stack([random_vector(hm.n, 400 + r) for r in range(nrhs)])
     Xt = torch.as_tensor(X, device=cuda)
     Yt = torch.as_tensor(Y0, device=cuda).clone()
-    print(f"[gpu] bfp mrhs nrhs={nrhs}", flush=True)
+    print(f"[gpu] {prec} mrhs nrhs={nrhs}", flush=True)
     plan.apply(0.5, N.OP_NORMAL, Xt, Yt)
     torch.cuda.synchronize()
     Yr = Y0.copy()
@@ -102,18 +110,19 @@ def test_multi_rhs(cuda, sphere16k, nrhs):
         assert rel(got[r] - Y0[r], Yr[r] - Y0[r]) <= TOL_FP32ACC, r
 
 
-def test_grad(cuda):
-    """Adjoint of a block floating point plan: A^T r against the oracle's
-    transposed product on the stored factors, factor gradients against their
-    closed forms (V^T x and U^T r accumulate in FP32 inside a panel)."""
+@pytest.mark.parametrize("prec", ["bfp", "afl"])
+def test_grad(cuda, prec):
+    """Adjoint of a packed plan: A^T r against the oracle's transposed
+    product on the stored factors, factor gradients against their closed
+    forms (V^T x and U^T r accumulate in FP32 inside a panel)."""
     import torch
     hm = HMatrix(MATERN_TLR.with_(n=4096))
-    plan = Plan.from_hmatrix(hm, precision="bfp", eps=1e-6)
+    plan = Plan.from_hmatrix(hm, precision=prec, eps=1e-6)
     n = hm.n
     x = torch.as_tensor(random_vector(n, 41, gaussian=True), device=cuda)
     r = torch.as_tensor(random_vector(n, 42, gaussian=True), device=cuda)
     gx = torch.zeros_like(x)
-    print(f"[gpu] bfp grad n={n}", flush=True)
+    print(f"[gpu] {prec} grad n={n}", flush=True)
     garena = plan.grad(x, r, gx)
     torch.cuda.synchronize()
     deq = dequantized_arena(plan, hm)

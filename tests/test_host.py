@@ -105,20 +105,25 @@ def test_valr_chooser_and_quantisation(sphere, eps):
     assert counts == p.info()["cols_by_format"]
 
 
-def _bfp_expected(ref, f):
-    if f >= N.FMT_B8:
+def _packed_expected(ref, f):
+    if N.FMT_B8 <= f <= N.FMT_B25:
         return oracle.bfp_round_column(ref, N.FMT_BITS[f])
+    if f >= N.FMT_A12:
+        return oracle.afl_round_column(ref, N.FMT_BITS[f])
     return ref.astype(np.float32).astype(np.float64) if f == N.FMT_F32 else ref
 
 
+@pytest.mark.parametrize("prec", ["bfp", "afl"])
 @pytest.mark.parametrize("eps", [1e-4, 1e-6, 1e-8])
-def test_bfp_chooser_and_quantisation(sphere, eps):
-    """Block floating point plans: every column's width follows the oracle's
-    chooser (U and V of a low-rank column share one width within the column's
-    VALR tolerance, dense columns within 2^-(rate(eps)+1)), and every stored
-    value equals the oracle's rounding with one step per 64-row panel column,
-    bit for bit."""
-    p = Plan.from_hmatrix(sphere, precision="bfp", eps=eps, device=-1)
+def test_packed_chooser_and_quantisation(sphere, prec, eps):
+    """Block floating point and AFL plans: every column's width follows the
+    oracle's chooser (U and V of a low-rank column share one width within
+    the column's VALR tolerance, dense columns within 2^-(rate(eps)+1)), and
+    every stored value equals the oracle's rounding with one step (BFP) or
+    scale (AFL) per 64-row panel column, bit for bit."""
+    choose = oracle.bfp_format if prec == "bfp" else oracle.afl_format
+    first, last = (N.FMT_B8, N.FMT_B25) if prec == "bfp" else (N.FMT_A12, N.FMT_A24)
+    p = Plan.from_hmatrix(sphere, precision=prec, eps=eps, device=-1)
     ar = [p.arena_numpy(f) for f in range(N.NFMT)]
     counts = [0] * N.NFMT
     rng = np.random.default_rng(5)
@@ -132,13 +137,13 @@ def test_bfp_chooser_and_quantisation(sphere, eps):
             D0 = sphere.arena[blk["off_a"]:blk["off_a"] + m * n].reshape(n, m).T
             D, _ = p.unpack_block(b, ar)
             for j in range(n):
-                f = oracle.bfp_format([D0[:, j]], 2.0 ** -(rate_eps + 1), rate_eps)
-                assert np.array_equal(D[:, j], _bfp_expected(D0[:, j], f)), (b, j, f)
+                f = choose([D0[:, j]], 2.0 ** -(rate_eps + 1), rate_eps)
+                assert np.array_equal(D[:, j], _packed_expected(D0[:, j], f)), (b, j, f)
             continue
         U0 = sphere.arena[blk["off_a"]:blk["off_a"] + m * k].reshape(k, m).T
         V0 = sphere.arena[blk["off_b"]:blk["off_b"] + n * k].reshape(k, n).T
         S = sphere.arena[blk["off_s"]:blk["off_s"] + k]
-        fm = [oracle.bfp_format([U0[:, l], V0[:, l]], u, oracle.eps_to_rate(u))
+        fm = [choose([U0[:, l], V0[:, l]], u, oracle.eps_to_rate(u))
               for l, u in enumerate(oracle.valr_tolerances(S, eps))]
         for f in fm:
             counts[f] += 1
@@ -146,13 +151,15 @@ def test_bfp_chooser_and_quantisation(sphere, eps):
             continue
         U, V = p.unpack_block(b, ar)
         for l, f in enumerate(fm):
-            assert np.array_equal(U[:, l], _bfp_expected(U0[:, l], f)), (b, l, f)
-            assert np.array_equal(V[:, l], _bfp_expected(V0[:, l], f)), (b, l, f)
+            assert np.array_equal(U[:, l], _packed_expected(U0[:, l], f)), (b, l, f)
+            assert np.array_equal(V[:, l], _packed_expected(V0[:, l], f)), (b, l, f)
     info = p.info()
     assert counts == info["cols_by_format"]
-    # the block floating point widths carry the columns, and the plan is
+    # the packed widths carry the columns (AFL's widest code has 18 mantissa
+    # bits, short of the tightest columns at eps = 1e-8), and the plan is
     # smaller than the mixed hardware-format plan at the same eps
-    assert sum(info["cols_by_format"][N.FMT_B8:]) > 0.8 * sum(counts)
+    share = 0.5 if (prec == "afl" and eps < 1e-7) else 0.8
+    assert sum(info["cols_by_format"][first:last + 1]) > share * sum(counts)
     q = Plan.from_hmatrix(sphere, precision="valr", eps=eps, device=-1)
     assert info["factor_bytes"] < q.info()["factor_bytes"]
     # bytes of each arena: 16 per row of 128 // bits values
@@ -160,19 +167,24 @@ def test_bfp_chooser_and_quantisation(sphere, eps):
         assert info["bytes_by_format"][f] == p.arena_elems(f) // (128 // N.FMT_BITS[f]) * 16
 
 
-def test_bfp_force_format_and_errors(sphere):
+def test_packed_force_format_and_errors(sphere):
     """force_format stores every low-rank column in one width; bad options
     are rejected."""
+    names = ("bfp8", "bfp16", "bfp25", "afl12", "afl16", "afl24")
     for f in range(N.FMT_B8, N.NFMT):
-        p = Plan.from_hmatrix(sphere, precision="bfp", eps=1e-6, device=-1,
-                              force_format=("bfp8", "bfp16", "bfp25")[f - N.FMT_B8])
+        prec = "bfp" if f <= N.FMT_B25 else "afl"
+        p = Plan.from_hmatrix(sphere, precision=prec, eps=1e-6, device=-1, force_format=names[f - N.FMT_B8])
         c = p.info()["cols_by_format"]
         assert c[f] == sum(c) > 0
+    for prec in ("bfp", "afl"):
+        with pytest.raises(RuntimeError):
+            Plan.from_hmatrix(sphere, precision=prec, eps=0.0, device=-1)
+        # BF16 / FP8 columns may not sit beside packed dense leaves
+        with pytest.raises(RuntimeError):
+            Plan.from_hmatrix(sphere, precision=prec, eps=1e-6, device=-1, force_format="bf16")
+    # nor block floating point beside AFL
     with pytest.raises(RuntimeError):
-        Plan.from_hmatrix(sphere, precision="bfp", eps=0.0, device=-1)
-    # BF16 / FP8 columns may not sit beside block floating point dense leaves
-    with pytest.raises(RuntimeError):
-        Plan.from_hmatrix(sphere, precision="bfp", eps=1e-6, device=-1, force_format="bf16")
+        Plan.from_hmatrix(sphere, precision="afl", eps=1e-6, device=-1, force_format="bfp16")
 
 
 def test_dense_and_datasize_valr(sphere):
