@@ -13,7 +13,9 @@ the plain FP64 plan is timed beside it, both with factors resident in HBM.
 A third plan stores every column in block floating point (values of 8, 16 or 25
 bits with a power-of-two step per panel column, width chosen per column so
 that the column's 2-norm error stays within the same VALR budget); its
-figures are reported beside the mixed-precision ones under "bfp".
+figures are reported beside the mixed-precision ones under "bfp".  A fourth
+stores them in adaptive float codes of 12, 16 or 24 bits (sign, 5 exponent
+bits, truncated mantissa, an FP32 scale per panel column) under "afl".
 The relative error of the mixed-precision result is taken against the FP64
 GPU result (which the full-size parity tests hold to <= 1e-13 of the FP64
 oracle on this matrix).  The CPU baseline is the oracle's restatement of the
@@ -54,9 +56,10 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=1, help="right-hand sides per call")
     ap.add_argument("--op", choices=["normal", "transposed"], default="normal",
                     help="product timed (the square sphere matrix keeps the same sizes)")
-    ap.add_argument("--precision", choices=["both", "fp64", "valr", "bfp"], default="both",
+    ap.add_argument("--precision", choices=["both", "fp64", "valr", "bfp", "afl"], default="both",
                     help="time only one plan (profiling runs); the JSON line needs fp64 and valr")
     ap.add_argument("--no-bfp", action="store_true", help="skip the block floating point plan")
+    ap.add_argument("--no-afl", action="store_true", help="skip the adaptive float plan")
     ap.add_argument("--mrhs", type=int, default=32,
                     help="also time each plan with this many right-hand sides per call (0: skip)")
     ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
@@ -180,7 +183,7 @@ def main() -> int:
     log(f"device copy: {copy_gbs:.0f} GB/s (read + write of {args.copy_gib:g} GiB)")
 
     ms64 = msmp = None
-    m64 = mmp = mbf = None
+    m64 = mmp = None
     ph64 = phmp = np.zeros(3)
     i64 = imp = None
     y64 = ymp = None
@@ -210,21 +213,23 @@ def main() -> int:
         mmp = time_mrhs(pmp)
         pmp.close()
 
-    # ---- block floating point plan ----
-    bfp = None
-    if args.precision == "bfp" or (args.precision == "both" and not args.no_bfp):
+    # ---- packed plans: block floating point, adaptive float ----
+    packed = {}
+    for fam, skip in (("bfp", args.no_bfp), ("afl", args.no_afl)):
+        if not (args.precision == fam or (args.precision == "both" and not skip)):
+            continue
         t0 = time.time()
-        pbf = Plan.from_hmatrix(hm, precision="bfp", eps=args.eps)
-        ibf = pbf.info()
-        log(f"bfp plan: {ibf['factor_bytes'] / 1e9:.2f} GB cols/format {ibf['cols_by_format']} "
+        pk = Plan.from_hmatrix(hm, precision=fam, eps=args.eps)
+        ipk = pk.info()
+        log(f"{fam} plan: {ipk['factor_bytes'] / 1e9:.2f} GB cols/format {ipk['cols_by_format']} "
             f"in {time.time() - t0:.1f} s")
-        msbf, phbf = time_plan(pbf, args.steps, args.warmup)
-        ybf = result(pbf)
-        log(f"bfp: {msbf:.3f} ms/MVM  phases {phbf.round(4).tolist()} ms")
-        mbf = time_mrhs(pbf)
-        pbf.close()
+        mspk, phpk = time_plan(pk, args.steps, args.warmup)
+        ypk = result(pk)
+        log(f"{fam}: {mspk:.3f} ms/MVM  phases {phpk.round(4).tolist()} ms")
+        mpk = time_mrhs(pk)
+        pk.close()
         torch.cuda.empty_cache()
-        bfp = {"ms_per_step": msbf, "info": ibf, "phases": phbf, "y": ybf}
+        packed[fam] = {"ms_per_step": mspk, "info": ipk, "phases": phpk, "y": ypk, "mrhs": mpk}
 
     if args.precision != "both":
         log("single-precision profiling run: no JSON line")
@@ -305,19 +310,19 @@ def main() -> int:
                      "algorithmic_bytes": r_bytes},
         "cpu_baseline": cpu,
     }
-    if bfp is not None:
-        db = (bfp["y"] - y64).reshape(R, -1)
-        err_bf = float(max(np.linalg.norm(db[j]) / np.linalg.norm(y64.reshape(R, -1)[j]) for j in range(R)))
-        ib = bfp["info"]
+    for fam, pk in packed.items():
+        db = (pk["y"] - y64).reshape(R, -1)
+        err_pk = float(max(np.linalg.norm(db[j]) / np.linalg.norm(y64.reshape(R, -1)[j]) for j in range(R)))
+        ib = pk["info"]
         bb = ib["datasize"] + 8.0 * 2 * (R - 1) * (n + n)
-        bf = {"mvm_per_s": 1000.0 * R / bfp["ms_per_step"], "ms_per_step": bfp["ms_per_step"],
-              "speedup_vs_fp64": ms64 / bfp["ms_per_step"], "rel_err_vs_fp64_gpu": err_bf,
-              "bytes_per_mvm": bb, "hbm_gbs": bb / (bfp["ms_per_step"] * 1e-3) / 1e9,
-              "phases_ms": bfp["phases"].tolist(), "cols_by_format": ib["cols_by_format"]}
-        if R == 1 and OP == N.OP_NORMAL and bfp["phases"][2] > 0:
+        bf = {"mvm_per_s": 1000.0 * R / pk["ms_per_step"], "ms_per_step": pk["ms_per_step"],
+              "speedup_vs_fp64": ms64 / pk["ms_per_step"], "rel_err_vs_fp64_gpu": err_pk,
+              "bytes_per_mvm": bb, "hbm_gbs": bb / (pk["ms_per_step"] * 1e-3) / 1e9,
+              "phases_ms": pk["phases"].tolist(), "cols_by_format": ib["cols_by_format"]}
+        if R == 1 and OP == N.OP_NORMAL and pk["phases"][2] > 0:
             rb = ib["bytes_rpass"] + 8.0 * (n + n)
-            bf["rpass_gbs"] = rb / (bfp["phases"][2] * 1e-3) / 1e9
-        out["bfp"] = bf
+            bf["rpass_gbs"] = rb / (pk["phases"][2] * 1e-3) / 1e9
+        out[fam] = bf
     if RM > 1 and m64 is not None and mmp is not None:
         # worst relative error over the right-hand sides against the FP64 plan
         def worst(m):
@@ -325,12 +330,14 @@ def main() -> int:
         mr = {"nrhs": RM, "fp64_ms": m64[0], "valr_ms": mmp[0], "speedup_valr": m64[0] / mmp[0],
               "rel_err_valr": worst(mmp), "h_mvm_per_s_fp64": 1000.0 * RM / m64[0],
               "h_mvm_per_s_valr": 1000.0 * RM / mmp[0]}
-        if bfp is not None and mbf is not None:
-            mr.update({"bfp_ms": mbf[0], "speedup_bfp": m64[0] / mbf[0], "rel_err_bfp": worst(mbf),
-                       "h_mvm_per_s_bfp": 1000.0 * RM / mbf[0]})
+        for fam, pk in packed.items():
+            mpk = pk["mrhs"]
+            if mpk is not None:
+                mr.update({f"{fam}_ms": mpk[0], f"speedup_{fam}": m64[0] / mpk[0], f"rel_err_{fam}": worst(mpk),
+                           f"h_mvm_per_s_{fam}": 1000.0 * RM / mpk[0]})
         out["mrhs"] = mr
         log(f"{RM} rhs: fp64 {m64[0]:.3f} ms, valr {mmp[0]:.3f} ms"
-            + (f", bfp {mbf[0]:.3f} ms" if "bfp_ms" in mr else ""))
+            + "".join(f", {fam} {mr[fam + '_ms']:.3f} ms" for fam in packed if f"{fam}_ms" in mr))
     print(json.dumps(out), flush=True)
     return 0
 
@@ -728,6 +735,16 @@ def bench_batch(args) -> int:
     flops = float(sum(i["flops"] * r for i, r in zip(infos, nrhs)))
     log(f"batch: {ms_batch:.3f} ms/step (graph replay {ms_graph:.3f}, plan by plan {ms_loop:.3f}); "
         f"schedule {chosen}; {nbytes / 1e9:.3f} GB and {flops / 1e9:.2f} GFlop per step")
+    # HBM traffic of the batched kernels from the committed rocprofv3 PMC
+    # summary of this workload (FETCH_SIZE + WRITE_SIZE per step), when it exists
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_batch_matern64.txt"
+    if tfile.exists():
+        traffic = 0.0
+        for line in tfile.read_text().splitlines():
+            f = line.split()
+            if len(f) >= 4 and not line.startswith("#") and f[0] == "batch":
+                traffic += float(f[-2]) + float(f[-1])
     out = {
         "metric": "batched H-MVM/s (64 Matern H-matrices, mixed nrhs)",
         "value": products / (ms_batch * 1e-3),
@@ -745,7 +762,10 @@ def bench_batch(args) -> int:
         "gflops": flops / (ms_batch * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": "whole step (batched kernels)",
                      "achieved": nbytes / (ms_batch * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": nbytes / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": nbytes / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": (f"profiles/{tfile.name} (rocprofv3 PMC FETCH_SIZE + WRITE_SIZE of the batched "
+                                        "kernels on this workload, copy-calibrated; not measured in this run)")
+                     if traffic is not None else None,
                      "algorithmic_bytes": nbytes},
         "cpu_baseline": cpu,
     }

@@ -218,6 +218,18 @@ __device__ inline bytep elem_addr(const void* arena, int64_t off) {
     default: { using T = Afl<24>; __VA_ARGS__; } break;              \
     }
 
+// Workgroup i of a launch is dispatched to XCD i % 8.  xcd_item(i, n, B)
+// reorders items so that each XCD runs consecutive items in runs of B (run
+// r goes to XCD r % 8): neighbouring items then share that XCD's L2, while
+// every XCD still takes runs from the whole range.  A bijection on [0, n):
+// the last n mod 8B items keep their places.
+__device__ inline int64_t xcd_item(int64_t i, int64_t n, int B) {
+    const int64_t full = n / (8 * B) * (8 * B);
+    if (i >= full) return i;
+    const int64_t x = i % 8, j = i / 8;
+    return ((j / B) * 8 + x) * B + (j % B);
+}
+
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

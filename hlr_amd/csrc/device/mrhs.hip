@@ -67,7 +67,9 @@ __global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const
                                                   int R) {
     extern __shared__ double zbuf[];               // 64 x kMmStride, sized at launch
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const RowPanel Rw = P.rp[blockIdx.x];
+    // runs of 32 consecutive row panels per XCD: the panels of one block row
+    // gather the same t and XT rows, which then come from that XCD's L2
+    const RowPanel Rw = P.rp[xcd_item(blockIdx.x, P.nrp, 32)];
     rpanel_mm<NTM, YT, FAM>(P, Rw, alpha, XTb, t, Y, ldy, R, zbuf, tid, wave, lane);
 }
 
