@@ -63,10 +63,12 @@ def main() -> int:
     ap.add_argument("--mrhs", type=int, default=32,
                     help="also time each plan with this many right-hand sides per call (0: skip)")
     ap.add_argument("--copy-gib", type=float, default=2.0, help="size of the device copy calibration")
-    ap.add_argument("--workload", choices=["sphere", "matern", "batch", "grad", "uniform"], default="sphere",
+    ap.add_argument("--workload", choices=["sphere", "matern", "batch", "grad", "uniform", "construct"],
+                    default="sphere",
                     help="sphere: 1/r, n = 262144 (default); matern: the plain FP64 drop-in on n = 65536; "
                          "batch: 64 Matern matrices, many right-hand sides; grad: loss and gradients (TLR); "
-                         "uniform: uniform-H and H^2 forms of the sphere matrix at n = 65536")
+                         "uniform: uniform-H and H^2 forms of the sphere matrix at n = 65536; "
+                         "construct: building the sphere matrix with its low-rank leaves on the GPU")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -81,6 +83,8 @@ def main() -> int:
         return bench_uniform(args)
     if args.workload == "grad":
         return bench_grad(args)
+    if args.workload == "construct":
+        return bench_construct(args)
 
     import numpy as np
     import torch
@@ -412,6 +416,83 @@ def bench_matern(args) -> int:
         out["speedup_vs_cpu"] = out["value"] / cpu["value"]
         out["speedup_vs_cpu_1core"] = out["value"] / cpu["value_1core"]
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_construct(args) -> int:
+    """Construction of the 1/r sphere matrix (n = 262144, leaf 64, eta 2, ACA
+    eps 1e-6): the host generator on all host cores (tree, blocks, dense
+    leaves and ACA + SVD truncation of the low-rank leaves) against the
+    split build (host: tree, blocks and dense leaves; GPU: ACA + truncation
+    of the low-rank leaves, hmvm_lowrank_build).  One step = one whole
+    construction; the split build's product is checked against the host
+    build's."""
+    import numpy as np
+    import torch
+
+    from hlr_amd import _native as N
+    from hlr_amd.hmatrix import INVR_SPHERE, HMatrix, random_vector
+    from hlr_amd.plan import Plan
+
+    dev = torch.device("cuda:0")
+    torch.zeros(1, device=dev)                      # runtime up before the clocks
+    prob = INVR_SPHERE.with_(n=args.n)
+    warm = min(args.warmup, 2)                      # whole constructions: a few suffice
+    for _ in range(warm):
+        HMatrix(prob.with_(n=8192), construct="device").close()
+    t0 = time.time()
+    hm = HMatrix(prob)
+    t_host = time.time() - t0
+    walls, devs, kers, cops = [], [], [], []
+    hd = None
+    for _ in range(max(1, min(args.steps, 3))):
+        if hd is not None:
+            hd.close()
+        t0 = time.time()
+        hd = HMatrix(prob, construct="device")
+        walls.append(time.time() - t0)
+        devs.append(hd.device_seconds)
+        kers.append(hd.device_kernel_seconds)
+        cops.append(hd.device_copy_seconds)
+    lr = hm.blocks["kind"] != N.BLOCK_DENSE
+    same = float((hd.blocks["rank"][lr] == hm.blocks["rank"][lr]).mean())
+    x = torch.as_tensor(random_vector(hm.n, 3), device=dev)
+    ys = []
+    for h in (hm, hd):
+        p = Plan.from_hmatrix(h, precision="fp64")
+        y = torch.zeros_like(x)
+        p.apply(1.0, N.OP_NORMAL, x, y)
+        torch.cuda.synchronize()
+        ys.append(y.cpu().numpy())
+        p.close()
+    err = float(np.linalg.norm(ys[1] - ys[0]) / np.linalg.norm(ys[0]))
+    wall = float(np.median(walls))
+    lr_bytes = float(sum(8.0 * (int(b["nrows"]) + int(b["ncols"]) + 1) * int(b["rank"]) for b in hd.blocks[lr]))
+    ncpu = len(os.sched_getaffinity(0))
+    log(f"construct: host {t_host:.2f} s ({hm.build_seconds:.2f} s in hlrgen_build, {ncpu} cores), "
+        f"split {wall:.2f} s (GPU part {np.median(devs):.3f} s: kernels {np.median(kers):.3f} s, factors to the "
+        f"host {np.median(cops):.3f} s), ranks equal on {same:.4f} of the low-rank leaves, "
+        f"product difference {err:.2e}")
+    out = {
+        "metric": "H-matrix construction time (1/r sphere, n=262144), low-rank leaves on the GPU",
+        "value": wall, "unit": "s", "n_gpus": 1, "steps": len(walls), "warmup": warm, "ms_per_step": wall * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded points on the sphere)",
+        "config": {"workload": f"1/r on unit sphere, n={hm.n}, leaf 64, eta 2, ACA eps 1e-6: tree, blocks and dense "
+                               "leaves on the host, ACA + SVD truncation of the low-rank leaves on the GPU",
+                   "parallelism": "single GPU"},
+        "gpu_seconds_lowrank": float(np.median(devs)),
+        "gpu_kernel_seconds": float(np.median(kers)), "factor_copy_seconds": float(np.median(cops)),
+        "lowrank_leaves": int(lr.sum()), "lowrank_factor_bytes": lr_bytes,
+        "ranks_equal_to_host_build": same, "product_rel_diff_vs_host_build": err,
+        "roofline": None,
+        "cpu_baseline": {"value": t_host, "unit": "s", "cores": ncpu, "kind": "port",
+                         "sample": f"the host generator (hlrgen_build, C++/OpenMP, all {ncpu} cores) building the "
+                                   "same matrix, ACA and truncation included (one build)"},
+    }
+    print(json.dumps(out), flush=True)
+    hd.close()
+    hm.close()
     return 0
 
 

@@ -66,7 +66,7 @@ class HlrgenConfig(C.Structure):
                 ("eta", C.c_double), ("eps", C.c_double), ("matern_ell", C.c_double),
                 ("matern_sigma", C.c_double), ("nn_diag", C.c_int32), ("lrformat", C.c_int32),
                 ("nthreads", C.c_int32), ("tlr_std_adm", C.c_int32), ("verbose", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("skip_lowrank", C.c_int32)]
 
 
 # numpy dtype matching hmvm_block
@@ -100,6 +100,20 @@ class HmvmUplanInfo(C.Structure):
                 ("nested", C.c_int32), ("levels", C.c_int32), ("basis_bytes", C.c_int64),
                 ("transfer_bytes", C.c_int64), ("coupling_bytes", C.c_int64), ("dense_bytes", C.c_int64),
                 ("flops", C.c_double), ("datasize", C.c_double)]
+
+
+KFN_LOG, KFN_LOG_GALERKIN, KFN_MATERN, KFN_INVR = 0, 1, 2, 3
+
+
+class HmvmKernelFn(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("n", C.c_int64), ("h", C.c_double),
+                ("ell", C.c_double), ("sigma", C.c_double), ("coords", C.c_void_p), ("perm", C.c_void_p)]
+
+
+# numpy dtype matching hmvm_lr_leaf
+def lr_leaf_dtype():
+    import numpy as np
+    return np.dtype([("row0", "<i8"), ("col0", "<i8"), ("nrows", "<i4"), ("ncols", "<i4")], align=True)
 
 
 class HmvmError(RuntimeError):
@@ -158,6 +172,12 @@ HMVM_SYMBOLS = [
     ("hmvm_uplan_destroy", _i32, [_vp]),
     ("hmvm_uplan_get_info", _i32, [_vp, C.POINTER(HmvmUplanInfo)]),
     ("hmvm_uplan_apply", _i32, [_vp, C.c_double, _i32, _vp, _vp, _vp]),
+    ("hmvm_lowrank_build", _i32, [C.POINTER(HmvmKernelFn), _vp, _i64, C.c_double, _i32, _i32, C.POINTER(_vp)]),
+    ("hmvm_lowrank_ranks", _i32, [_vp, _vp]),
+    ("hmvm_lowrank_copy", _i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i64]),
+    ("hmvm_lowrank_seconds", C.c_double, [_vp]),
+    ("hmvm_lowrank_timing", _i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("hmvm_lowrank_free", None, [_vp]),
     ("hmvm_error_string", C.c_char_p, [_i32]),
 ]
 

@@ -49,7 +49,8 @@ def build_hlrgen() -> Path:
 def build_hmvm() -> Path:
     LIBDIR.mkdir(exist_ok=True)
     dev = CSRC / "device"
-    srcs = ("plan_build.cpp", "hmfile.cpp", "hmvm.hip", "mrhs.hip", "batch.hip", "vecops.hip", "uniform.hip")
+    srcs = ("plan_build.cpp", "hmfile.cpp", "hmvm.hip", "mrhs.hip", "batch.hip", "vecops.hip", "uniform.hip",
+            "construct.hip")
     objs = [str(LIBDIR / (Path(src).stem + ".o")) for src in srcs]
 
     def compile_one(src: str, obj: str) -> None:

@@ -643,6 +643,8 @@ extern "C" int hlrgen_build(const hlrgen_config* cfgp, hlrgen_matrix** out) {
                         d.a.resize(m * nn);
                         for (int64_t j = 0; j < nn; ++j)
                             for (int64_t i = 0; i < m; ++i) d.a[j * m + i] = P.eval(R.first + i, C.first + j);
+                    } else if (cfg.skip_lowrank) {
+                        d.k = 0;
                     } else {
                         aca_block(P, R.first, m, C.first, nn, cfg.eps, std::min(m, nn), aca);
                         const int64_t k = truncate_lr(m, nn, aca.k, aca.U.data(), aca.V.data(), cfg.eps, Uo, So, Vo);

@@ -1,8 +1,11 @@
 """H-/TLR-matrices from the native generator (libhlrgen.so).
 
 `HMatrix` holds the flattened leaf list (in hlr::mul_vec visiting order) and
-the FP64 factor arena produced by hlrgen_build.  The named problems restate
-the workloads the product is measured on (kernel, geometry, sizes, seeds).
+the FP64 factor arena produced by hlrgen_build.  With construct="device" the
+host builds the cluster tree, the block tree and the dense leaves, and the
+low-rank leaves are built on the GPU (hmvm_lowrank_build: ACA and SVD
+truncation, one workgroup per leaf).  The named problems restate the
+workloads the product is measured on (kernel, geometry, sizes, seeds).
 """
 from __future__ import annotations
 
@@ -67,15 +70,18 @@ def batch_problems(count: int = 64, seed: int = 7, nmin: int = 2048, nmax: int =
 class HMatrix:
     """A generated H-matrix: leaves + FP64 arena (owned by the native side)."""
 
-    def __init__(self, problem: Problem, verbose: bool = False):
+    def __init__(self, problem: Problem, verbose: bool = False, construct: str = "host", device: int = 0):
+        if construct not in ("host", "device"):
+            raise ValueError(f"construct must be 'host' or 'device', not {construct!r}")
         lib = N.hlrgen()
         cfg = N.HlrgenConfig()
         lib.hlrgen_default_config(C.byref(cfg))
         for name, _ in N.HlrgenConfig._fields_:
-            if name in ("verbose", "reserved"):
+            if name in ("verbose", "skip_lowrank"):
                 continue
             setattr(cfg, name, getattr(problem, name))
         cfg.verbose = 1 if verbose else 0
+        cfg.skip_lowrank = 1 if construct == "device" else 0
         h = C.c_void_p()
         N.check(lib.hlrgen_build(C.byref(cfg), C.byref(h)), "hlrgen_build")
         self._h = h
@@ -92,6 +98,61 @@ class HMatrix:
         self.coords = np.ctypeslib.as_array(
             (C.c_double * (3 * self.n)).from_address(lib.hlrgen_coords(h))).reshape(self.n, 3).copy()
         self.build_seconds = float(lib.hlrgen_build_seconds(h))
+        self.construct = construct
+        self.device_seconds = None
+        if construct == "device":
+            self._build_lowrank_on_device(device)
+
+    def _build_lowrank_on_device(self, device: int) -> None:
+        """Low-rank leaves (left at rank 0 by the host) from hmvm_lowrank_build;
+        the arena grows by their factors in the host generator's layout (U,
+        V, S per leaf, each padded to 4 values; S folded into the factor
+        with fewer rows for HLRGEN_LR_UV)."""
+        p = self.problem
+        lr = np.nonzero(self.blocks["kind"] != N.BLOCK_DENSE)[0]
+        leaves = np.zeros(len(lr), dtype=N.lr_leaf_dtype())
+        leaves["row0"] = self.blocks["row_ofs"][lr]
+        leaves["col0"] = self.blocks["col_ofs"][lr]
+        leaves["nrows"] = self.blocks["nrows"][lr]
+        leaves["ncols"] = self.blocks["ncols"][lr]
+        coords = np.ascontiguousarray(self.coords)
+        perm = np.ascontiguousarray(self.perm)
+        fn = N.HmvmKernelFn(kind=p.kernel, n=self.n, h=1.0 / self.n, ell=p.matern_ell, sigma=p.matern_sigma,
+                            coords=coords.ctypes.data, perm=perm.ctypes.data)
+        lib = N.hmvm()
+        h = C.c_void_p()
+        N.check(lib.hmvm_lowrank_build(C.byref(fn), leaves.ctypes.data if len(lr) else None, len(lr), p.eps, 1,
+                                       device, C.byref(h)), "hmvm_lowrank_build")
+        try:
+            ranks = np.zeros(len(lr), dtype=np.int32)
+            N.check(lib.hmvm_lowrank_ranks(h, ranks.ctypes.data), "hmvm_lowrank_ranks")
+            m = self.blocks["nrows"][lr].astype(np.int64)
+            n = self.blocks["ncols"][lr].astype(np.int64)
+            k = ranks.astype(np.int64)
+            pad4 = lambda v: (v + 3) & ~np.int64(3)
+            sizes = np.stack([pad4(m * k), pad4(n * k), pad4(k)], axis=1)
+            base = self.arena.size
+            ends = base + np.cumsum(sizes.reshape(-1))
+            starts = (ends - sizes.reshape(-1)).reshape(-1, 3)
+            arena = np.zeros(int(ends[-1]) if len(lr) else base)
+            arena[:base] = self.arena
+            off_a = np.ascontiguousarray(starts[:, 0])
+            off_b = np.ascontiguousarray(starts[:, 1])
+            off_s = np.ascontiguousarray(starts[:, 2])
+            fold = 1 if p.lrformat == N.LR_UV else 0
+            N.check(lib.hmvm_lowrank_copy(h, off_a.ctypes.data, off_b.ctypes.data, off_s.ctypes.data, fold,
+                                          arena.ctypes.data, arena.size), "hmvm_lowrank_copy")
+            self.device_seconds = float(lib.hmvm_lowrank_seconds(h))
+            ks, cs = C.c_double(), C.c_double()
+            N.check(lib.hmvm_lowrank_timing(h, C.byref(ks), C.byref(cs)), "hmvm_lowrank_timing")
+            self.device_kernel_seconds, self.device_copy_seconds = float(ks.value), float(cs.value)
+        finally:
+            lib.hmvm_lowrank_free(h)
+        self.blocks["rank"][lr] = ranks
+        self.blocks["off_a"][lr] = off_a
+        self.blocks["off_b"][lr] = off_b
+        self.blocks["off_s"][lr] = off_s
+        self.arena = arena
 
     # the arena view is only valid while the native matrix lives
     def close(self) -> None:

@@ -64,7 +64,9 @@ typedef struct hlrgen_config {
     int32_t  nthreads;      /* 0 = OpenMP default                                   */
     int32_t  tlr_std_adm;   /* TLR: 1 = standard admissibility, 0 = off-diagonal    */
     int32_t  verbose;
-    int32_t  reserved;
+    int32_t  skip_lowrank;  /* 1: leave the low-rank leaves at rank 0 (their
+                               factors are built elsewhere, e.g. on the device
+                               by hmvm_lowrank_build); dense leaves as always */
 } hlrgen_config;
 
 typedef struct hlrgen_matrix hlrgen_matrix;

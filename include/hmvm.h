@@ -529,6 +529,61 @@ int hmvm_uplan_get_info(const hmvm_uplan* plan, hmvm_uplan_info* info);
 int hmvm_uplan_apply(const hmvm_uplan* plan, double alpha, int op,
                      const double* x, double* y, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Low-rank leaves built on the device                                       */
+/* ------------------------------------------------------------------------ */
+
+/* kernel functions a(i, j) of the generator (include/hlrgen.h kernels) */
+enum {
+    HMVM_KFN_LOG          = 0,   /* log|x_i - x_j|                                  */
+    HMVM_KFN_LOG_GALERKIN = 1,   /* Galerkin log entries on a uniform 1D grid       */
+    HMVM_KFN_MATERN       = 2,   /* sigma^2 (1 + s) exp(-s), s = sqrt(3) r / ell    */
+    HMVM_KFN_INVR         = 3    /* 1 / |x_i - x_j|                                 */
+};
+
+typedef struct hmvm_kernel_fn {
+    int32_t        kind;         /* HMVM_KFN_*                                     */
+    int32_t        pad_;
+    int64_t        n;            /* points                                         */
+    double         h;            /* 1D cell width (log kernels)                    */
+    double         ell, sigma;   /* Matern length scale, standard deviation        */
+    const double*  coords;       /* host, n x 3 in internal order                  */
+    const int64_t* perm;         /* host, internal -> external (Galerkin log)      */
+} hmvm_kernel_fn;
+
+typedef struct hmvm_lr_leaf {
+    int64_t row0, col0;          /* first internal row / column                    */
+    int32_t nrows, ncols;        /* disjoint row and column ranges                 */
+} hmvm_lr_leaf;
+
+typedef struct hmvm_lowrank hmvm_lowrank;
+
+/*
+ * Low-rank factors of the kernel blocks `leaves` on device `device`: ACA
+ * with maximal-pivot search to relative accuracy eps, then (truncate != 0)
+ * SVD truncation to eps: U (m x k) and V (n x k) orthonormal, S descending.
+ * Without truncation U and V are the ACA factors and S is all ones.
+ * Replaces the leaf-wise build of seq/matrix.hh:51-176 with
+ * bem/aca.hh:59-78 (approx/aca.hh:117-434, approx/svd.hh:90-209); the host
+ * generator's hlrgen_build runs the same steps on the CPU.  ACA ranks above
+ * 128 give HMVM_ERR_UNSUPPORTED; a leaf holding a diagonal entry,
+ * HMVM_ERR_BAD_BLOCK.  The factors are returned in host memory.
+ */
+int hmvm_lowrank_build(const hmvm_kernel_fn* fn, const hmvm_lr_leaf* leaves, int64_t nleaves, double eps,
+                       int truncate, int device, hmvm_lowrank** out);
+int hmvm_lowrank_ranks(const hmvm_lowrank* lr, int32_t* ranks);
+/* copy leaf i's U to arena[off_a[i]..], V to arena[off_b[i]..] and (off_s
+   not NULL, off_s[i] >= 0) S to arena[off_s[i]..], column-major; fold != 0
+   multiplies S into the factor with fewer rows (V on ties), as
+   HLRGEN_LR_UV stores it */
+int hmvm_lowrank_copy(const hmvm_lowrank* lr, const int64_t* off_a, const int64_t* off_b, const int64_t* off_s,
+                      int fold, double* arena, int64_t arena_len);
+/* device time of the build (seconds), from HIP events around it */
+double hmvm_lowrank_seconds(const hmvm_lowrank* lr);
+/* of which the kernels, and the copy of the factors to the host */
+int hmvm_lowrank_timing(const hmvm_lowrank* lr, double* kernel_seconds, double* copy_seconds);
+void hmvm_lowrank_free(hmvm_lowrank* lr);
+
 const char* hmvm_error_string(int code);
 
 #ifdef __cplusplus
