@@ -370,9 +370,9 @@ int hmvm_batch_create(const hmvm_plan* const* plans, int nplans, const int* nrhs
     }
     b->lds[K_XT] = mm_lds_bytes(maxRp);
     b->lds[K_V1] = size_t(kWaves) * kPackXs * sizeof(double);
-    for (int k = K_VM1; k <= K_VM4; ++k) b->lds[k] = mm_lds_bytes(maxRp);
+    for (int k = K_VM1; k <= K_VM4; ++k) b->lds[k] = mm_lds_bytes(16 * (k - K_VM1 + 1));
     b->lds[K_R1] = size_t(kChunkGroups) * 16 * sizeof(float);
-    for (int k = K_RM1; k <= K_RM4; ++k) b->lds[k] = mm_lds_bytes(maxRp);
+    for (int k = K_RM1; k <= K_RM4; ++k) b->lds[k] = mm_lds_bytes(16 * (k - K_RM1 + 1));
 
     int rc;
     if ((rc = dev_alloc(b.get(), &b->d_mats, size_t(nplans))) != HMVM_OK) { release(b.get()); return rc; }

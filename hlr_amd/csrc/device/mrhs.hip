@@ -65,7 +65,7 @@ template <int NTM, typename YT, int FAM>
 __global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const double* __restrict__ XTb,
                                                   const double* __restrict__ t, YT* __restrict__ Y, int64_t ldy,
                                                   int R) {
-    extern __shared__ double zbuf[];               // 64 x kMmStride, sized at launch
+    extern __shared__ double zbuf[];               // 64 x mm_stride<NTM>(), sized at launch
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // runs of 32 consecutive row panels per XCD: the panels of one block row
     // gather the same t and XT rows, which then come from that XCD's L2

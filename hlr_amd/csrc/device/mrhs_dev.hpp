@@ -49,7 +49,7 @@ constexpr int mm_jb(int W) { return W % 5 == 0 ? 5 : W % 4 == 0 ? 4 : W % 3 == 0
 // Wave w takes groups 16 w .. 16 w + 15 of the segment; lane n + 16 k loads
 // row 4 s + k of group 16 w + n with one 16-byte load, whose element j is
 // B[k][n] of MFMA j (column W (16 w + n) + j).  A[m][k] = X(row 4 s + k,
-// rhs m) comes from the x slice in LDS (64 rows x kMmStride).
+// rhs m) comes from the x slice in LDS (64 rows x mm_stride<NTM>()).
 // Accumulators: JB elements x up to 4 rhs tiles; formats with W > JB make
 // W / JB passes over the segment.
 // ---------------------------------------------------------------------------
@@ -65,7 +65,7 @@ constexpr int mm_jb(int W) { return W % 5 == 0 ? 5 : W % 4 == 0 ? 4 : W % 3 == 0
 template <typename XT>
 __device__ inline void xtrans_tile(const XT* __restrict__ X, int64_t ldx, int64_t n, int R, double* __restrict__ XTb,
                                    int64_t i0, double* tile, int tid) {
-    constexpr int TS = kMaxRp + 1;                       // [64][TS]: index, rhs
+    const int TS = ((R + 15) & ~15) + 1;                 // [64][TS]: index, rhs (LDS of mm_lds_bytes(Rp))
     for (int idx = tid; idx < 64 * R; idx += kThr) {
         const int i = idx & 63, r = idx >> 6;
         tile[i * TS + r] = (i0 + i < n) ? double(X[int64_t(r) * ldx + i0 + i]) : 0.0;
@@ -78,11 +78,14 @@ __device__ inline void xtrans_tile(const XT* __restrict__ X, int64_t ldx, int64_
 }
 
 // LDS of one matrix-core workgroup: 64 rows (V pass) or 64 columns (R pass)
-// of up to kMaxRp right-hand sides, with the compile-time row stride kMmStride
-// (odd, so that neighbouring rows start in different banks; being constant,
-// every operand read is one base register plus an immediate offset)
-constexpr int kMmStride = kMaxRp + 1;
-__host__ __device__ inline size_t mm_lds_bytes(int) { return size_t(64) * kMmStride * sizeof(double); }
+// of Rp = 16 NTM right-hand sides, with the row stride 16 NTM + 1 (odd, so
+// that neighbouring rows start in different banks; a compile-time constant,
+// so that every operand read is one base register plus an immediate
+// offset).  Sized by the tile count, not by the largest one: a workgroup of
+// one or two tiles takes a quarter or half of the LDS of four, and more of
+// them share a CU (32 right-hand sides, mixed plan: 2.05 -> 1.80 ms).
+template <int NTM> constexpr int mm_stride() { return 16 * NTM + 1; }
+__host__ __device__ inline size_t mm_lds_bytes(int Rp) { return size_t(64) * (Rp + 1) * sizeof(double); }
 
 template <int NTM, typename T>
 __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const double* __restrict__ XTb, int R,
@@ -96,7 +99,7 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
     const int m = sg.m;                                  // up to kUnitRows, staged 64 rows at a time
     const int Rp = (R + 15) & ~15;
     const int NT = Rp / 16;
-    constexpr int XS = kMmStride;
+    constexpr int XS = mm_stride<NTM>();
     A* xs = reinterpret_cast<A*>(xsraw);                 // [64][XS]: row, rhs
     constexpr int RS = 16 * NTM;                         // staged rhs slots (constant divisor)
     // rows i0 .. i0 + 63 of the segment's x slice
@@ -256,7 +259,7 @@ __device__ inline void psum_mm_item2(const DevPlan& D, const double* __restrict_
 
 // R pass of one chunk: Y(i, r) += sum_c R(i, c) Z(c, r) with M = rows (16
 // per wave), N = right-hand sides, K = columns.  Z is staged 64 columns at
-// a time in LDS (64 x kMmStride); lane i + 16 k loads row 16 w + i of group
+// a time in LDS (64 x mm_stride<NTM>()); lane i + 16 k loads row 16 w + i of group
 // g0 + k with one 16-byte load, whose element j is A[i][k] of MFMA j
 // (column W (g0 + k) + j), and B[k][n] = Z(W (g0 + k) + j, rhs n).
 template <int NTM, typename T, bool kUnroll>
@@ -277,7 +280,7 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     constexpr int NB = MmSteps<W>::value;
     constexpr int RS = 16 * NTM;                         // staged rhs slots (constant divisor)
     A* zs = static_cast<A*>(zraw);                       // [64][ZS]: column, rhs
-    constexpr int ZS = kMmStride;
+    constexpr int ZS = mm_stride<NTM>();
     const int NT = Rp / 16;
     const int ncol = Ch.ngroups * W;
     const int row = 16 * wave + (lane & 15);
