@@ -552,8 +552,15 @@ extern "C" int hmvm_lowrank_build(const hmvm_kernel_fn* fn, const hmvm_lr_leaf* 
     R->truncate = truncate ? 1 : 0;
     Guard g(device);
     if (!g.ok) return HMVM_ERR_DEVICE;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return HMVM_ERR_DEVICE;
+    // e0 / e1: the whole build; k0 / k1: each chunk's leaf kernel
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (hipEvent_t& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            for (hipEvent_t& f : ev)
+                if (f) (void)hipEventDestroy(f);
+            return HMVM_ERR_DEVICE;
+        }
+    hipEvent_t e0 = ev[0], e1 = ev[1], k0 = ev[2], k1 = ev[3];
     (void)hipEventRecord(e0, nullptr);
     // problem data on the device
     const int64_t np = fn->n;
@@ -580,8 +587,6 @@ extern "C" int hmvm_lowrank_build(const hmvm_kernel_fn* fn, const hmvm_lr_leaf* 
     // done; then one host buffer of the exact size takes them
     std::vector<std::pair<double*, int64_t>> outs;  // device buffer, values
     int64_t total = 0;
-    hipEvent_t k0, k1;
-    if (hipEventCreate(&k0) != hipSuccess || hipEventCreate(&k1) != hipSuccess) return HMVM_ERR_DEVICE;
     double kernel_ms = 0;
     double* dws = nullptr;
     Leaf* dleaf = nullptr;
