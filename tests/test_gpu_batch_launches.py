@@ -112,12 +112,15 @@ def test_batch_mixed_precision(cuda, members):
     b.close()
 
 
-def test_batch_block_floating_point(cuda, members):
+@pytest.mark.parametrize("prec", ["bfp", "afl"])
+def test_batch_packed_formats(cuda, members, prec):
+    """Members stored in block floating point or AFL: the batch runs the
+    family's own kernel instantiation and stays within eps of the oracle."""
     hms, nrhs = members
-    plans = [Plan.from_hmatrix(hm, precision="bfp", eps=1e-6) for hm in hms]
+    plans = [Plan.from_hmatrix(hm, precision=prec, eps=1e-6) for hm in hms]
     b = Batch(plans, nrhs)
     xs, ys = _vectors(hms, nrhs, cuda, 10)
-    print(f"[gpu] batch bfp n={[hm.n for hm in hms]} nrhs={nrhs}", flush=True)
+    print(f"[gpu] batch {prec} n={[hm.n for hm in hms]} nrhs={nrhs}", flush=True)
     b.apply(xs, ys)
     b.check()
     _check(hms, nrhs, xs, ys, [1.0] * len(hms), 1e-6)
