@@ -294,7 +294,12 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     // a plain loop, which needs fewer registers there.
     constexpr bool kFixedR = (kThr % RS) == 0;
     constexpr int NI = (SC * RS + kThr - 1) / kThr;
-    constexpr int ZB = NI < 4 ? NI : 4;
+    // values per round: up to 16 (one or two rounds per block) in the
+    // FP64-only kernel, which has the registers for them (batch step 13.0 ->
+    // 12.5 ms); 4 in the kernels of the other families, whose registers
+    // are those of their widest decode path and which lose occupancy with
+    // more (32 right-hand sides, mixed plan 1.81 -> 2.11 ms at 16)
+    constexpr int ZB = NI < (kUnroll ? 16 : 4) ? NI : (kUnroll ? 16 : 4);
     const int c0 = tid / RS, r0 = tid % RS;
     for (int gb0 = 0; gb0 < Ch.ngroups; gb0 += GB) {
         const int64_t cb = int64_t(Ch.col0) + gb0 * W;
