@@ -262,7 +262,7 @@ __device__ inline void psum_mm_item2(const DevPlan& D, const double* __restrict_
 // a time in LDS (64 x mm_stride<NTM>()); lane i + 16 k loads row 16 w + i of group
 // g0 + k with one 16-byte load, whose element j is A[i][k] of MFMA j
 // (column W (g0 + k) + j), and B[k][n] = Z(W (g0 + k) + j, rhs n).
-template <int NTM, typename T, bool kUnroll>
+template <int NTM, typename T, int FAM>
 __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alpha, const double* __restrict__ XTb,
                           const double* __restrict__ t, int R, int Rp, void* zraw,
                           typename Mm<T>::V4 (&acc)[NTM], int tid, int wave, int lane) {
@@ -270,6 +270,7 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     using A = typename M::A;
     using E = Elem<T>;
     constexpr int W = M::W;
+    constexpr bool kUnroll = (FAM == kFamF64);           // the FP64-only kernel
     // groups per staged block: a multiple of the 4 groups of one MFMA step,
     // so that no step reads past the staged columns; the LDS of 64 FP64
     // columns holds 128 FP32 ones (formats decoded to FP32)
@@ -305,7 +306,50 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
         const int64_t cb = int64_t(Ch.col0) + gb0 * W;
         const int lim = min(SC, ncol - gb0 * W);         // columns of this block that exist
         __syncthreads();                                 // the MFMAs of the block before are done with zs
-        if constexpr (kFixedR) {
+        // The other families, for an even number of right-hand sides: a
+        // thread stages two neighbouring right-hand sides of a column with
+        // one 16-byte load (t and XT rows are 16-byte aligned then), so a
+        // block takes half the rounds: 8 values per round for the hardware
+        // formats, 4 for the packed ones, whose decode paths leave fewer
+        // registers.  One load from the selected row (x or t) keeps the
+        // registers of the scalar rounds (32 right-hand sides: mixed 1.81
+        // -> 1.70 ms, block floating point 2.18 -> 1.93 ms, adaptive
+        // floating point 2.30 -> 2.06 ms).
+        constexpr int RS2 = RS / 2;                      // threads per column
+        constexpr int NI2 = (SC * RS2 + kThr - 1) / kThr;
+        constexpr int ZB2 = NI2 < (FAM == kFamHw ? 8 : 4) ? NI2 : (FAM == kFamHw ? 8 : 4);
+        if (!kUnroll && (kThr % RS2) == 0 && (R % 2) == 0) {
+            const int c2 = tid / RS2, r2 = 2 * (tid % RS2);
+            auto round2 = [&](int j0) {
+                int32_t src[ZB2];
+                float sc[ZB2];
+#pragma unroll
+                for (int u = 0; u < ZB2; ++u) {
+                    const int c = c2 + (j0 + u) * (kThr / RS2);
+                    src[u] = (j0 + u < NI2 && c < lim) ? P.colsrc[cb + c] : kPadCol;
+                    if constexpr (E::kScaled) sc[u] = (j0 + u < NI2 && c < lim) ? P.zscale[cb + c] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < ZB2; ++u) {
+                    const int c = c2 + (j0 + u) * (kThr / RS2);
+                    const int32_t s = src[u];
+                    double2 z{0.0, 0.0};
+                    if (s != kPadCol && r2 < R) {
+                        // one load from a selected row: x (times alpha) or t
+                        const double* src2 = (s >= 0) ? XTb + int64_t(s) * R + r2 : t + int64_t(~s) * R + r2;
+                        z = *reinterpret_cast<const double2*>(src2);
+                        if (s >= 0) { z.x *= alpha; z.y *= alpha; }
+                        if constexpr (E::kScaled) { z.x *= double(sc[u]); z.y *= double(sc[u]); }
+                    }
+                    if (j0 + u < NI2 && c < SC) {
+                        zs[c * ZS + r2] = A(z.x);
+                        zs[c * ZS + r2 + 1] = A(z.y);
+                    }
+                }
+            };
+#pragma unroll 1
+            for (int j0 = 0; j0 < NI2; j0 += ZB2) round2(j0);
+        } else if constexpr (kFixedR) {
             // one round: ZB column-source loads, then ZB value loads
             auto round = [&](int j0) {
                 int32_t src[ZB];
@@ -400,7 +444,7 @@ __device__ inline void rpanel_mm(const DevPlan& P, const RowPanel& Rw, double al
     for (int i = 0; i < NTM; ++i) { acc64[i] = d4{0, 0, 0, 0}; acc32[i] = f4{0, 0, 0, 0}; }
     for (int ci = 0; ci < Rw.nchunks; ++ci) {
         const RChunk Ch = P.rchunks[Rw.chunk0 + ci];
-        HMVM_FMT_SWITCH_FAM(FAM, Ch.fmt, rchunk_mm<NTM, T, FAM == kFamF64>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf,
+        HMVM_FMT_SWITCH_FAM(FAM, Ch.fmt, rchunk_mm<NTM, T, FAM>(P, Ch, Rw.m, alpha, XTb, t, R, Rp, zbuf,
                                                             RAcc<T>::pick(acc64, acc32), tid, wave, lane))
     }
     // bring the FP64 tiles into the FP32 element order (row 4 kk + e), then
