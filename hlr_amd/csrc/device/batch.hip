@@ -100,17 +100,16 @@ __global__ __launch_bounds__(kThreads) void k_bvm(const BatchMat* __restrict__ m
 
 // partial sums of one member: items (entry, pair of right-hand sides) when R
 // is even, else (entry, right-hand side); the workspace is 256-byte aligned
+static_assert(kSumItems[1] <= kBsumItems * kThreads, "a sum unit is at most kBsumItems items per thread");
 __global__ __launch_bounds__(kThreads) void k_bps(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
-    const int64_t g1 = int64_t(w.first) + w.count;
-    if (M.R % 2 == 0) {
-        for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads)
-            psum_mm_item2(M.D, M.part, M.t, M.R, M.alpha, g);
-    } else {
-        for (int64_t g = w.first + threadIdx.x; g < g1; g += kThreads) psum_mm_item(M.D, M.part, M.t, M.R, M.alpha, g);
-    }
+    const int64_t g0 = int64_t(w.first) + threadIdx.x, g1 = int64_t(w.first) + w.count;
+    if (M.R % 2 == 0)
+        psum_mm_items<2, kBsumItems, kBsumRound>(M.D, M.part, M.t, M.R, M.alpha, g0, kThreads, g1);
+    else
+        psum_mm_items<1, kBsumItems, kBsumRound>(M.D, M.part, M.t, M.R, M.alpha, g0, kThreads, g1);
 }
 
 template <typename XT, typename YT, int FAM>

@@ -49,16 +49,12 @@ __global__ __launch_bounds__(kThr) void k_vfwd_mm(DevPlan P, const double* __res
     vseg_mm_dispatch<NTM, FAM>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
 }
 
+// one item (entry, NP right-hand sides) per thread
+template <int NP>
 __global__ __launch_bounds__(256) void k_psum_mm(DevPlan D, const double* __restrict__ Pb, double* __restrict__ t,
                                                  int R, double alpha) {
     const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g < D.ntent * R) psum_mm_item(D, Pb, t, R, alpha, g);
-}
-
-__global__ __launch_bounds__(256) void k_psum_mm2(DevPlan D, const double* __restrict__ Pb, double* __restrict__ t,
-                                                  int R, double alpha) {
-    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g < D.ntent * (R >> 1)) psum_mm_item2(D, Pb, t, R, alpha, g);
+    psum_mm_items<NP, 1, kPsumRound<NP>>(D, Pb, t, R, alpha, g, 0, D.ntent * (R / NP));
 }
 
 template <int NTM, typename YT, int FAM>
@@ -131,9 +127,9 @@ int launch_apply_mrhs(const DevPlan& D, int fam, double alpha, const void* x, in
     if (D.ntent > 0) {
         // pairs of right-hand sides when R is even and the workspace 16-byte aligned
         if (R % 2 == 0 && (reinterpret_cast<uintptr_t>(part) & 15) == 0)
-            k_psum_mm2<<<blocks(D.ntent * (R / 2), 256), 256, 0, st>>>(D, part, t, R, alpha);
+            k_psum_mm<2><<<blocks(D.ntent * (R / 2), 256), 256, 0, st>>>(D, part, t, R, alpha);
         else
-            k_psum_mm<<<blocks(D.ntent * R, 256), 256, 0, st>>>(D, part, t, R, alpha);
+            k_psum_mm<1><<<blocks(D.ntent * R, 256), 256, 0, st>>>(D, part, t, R, alpha);
     }
     if (D.nrp > 0) by_tiles(rpass);
     return hipGetLastError() == hipSuccess ? 0 : 2;

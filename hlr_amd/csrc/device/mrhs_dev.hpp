@@ -195,63 +195,109 @@ __device__ inline void vseg_mm_dispatch(const DevPlan& P, const VItem& sg, const
 // partial sums for R right-hand sides (entry-major, rhs innermost)
 // ---------------------------------------------------------------------------
 
-// t(e, r) = alpha S_e sum of the entry's partials in one pass: each segment
-// of kSegLen partials summed in order from zero, then the segment sums in
-// order (a fixed order, the same for every schedule and batch); a
-// segment's loads are issued together
-__device__ inline void psum_mm_item(const DevPlan& D, const double* __restrict__ Pb, double* __restrict__ t, int R,
-                                    double alpha, int64_t g) {
-    const int64_t e = g / R;
-    const int r = int(g % R);
-    const int32_t o = D.pt_off[e], k = D.tstride[e], cnt = D.pt_cnt[e];
-    const double* src = Pb + int64_t(o) * R + r;
-    const int64_t step = int64_t(k) * R;
-    double a = 0;
-    for (int32_t r0 = 0; r0 < cnt; r0 += kSegLen) {
-        const int32_t n = min(kSegLen, cnt - r0);
-        double v[kSegLen];
-#pragma unroll
-        for (int u = 0; u < kSegLen; ++u) v[u] = (u < n) ? src[(int64_t(r0) + u) * step] : 0.0;
-        double s = 0;
-#pragma unroll
-        for (int u = 0; u < kSegLen; ++u)
-            if (u < n) s += v[u];
-        a += s;
+// t(e, r .. r + NP - 1) = alpha S_e sum of the entry's partials, for NP = 1
+// or 2 adjacent right-hand sides (NP = 2: R even, one 16-byte load per
+// partial; P and t are 16-byte aligned and o R + r is even).  Each segment
+// of kSegLen partials is summed in order from zero, then the segment sums in
+// order: a fixed order, the same for every schedule, batch and NI.
+//
+// A thread takes up to NI items g0, g0 + stride, ... (those below g1): the
+// entry tables and the first partial of all of them are loaded together,
+// then each item's remaining partials in rounds of B loads.
+template <int NP>
+struct PsumVec {
+    double v[NP];
+};
+
+template <int NP>
+__device__ inline PsumVec<NP> psum_load(const double* p) {
+    PsumVec<NP> r;
+    if constexpr (NP == 2) {
+        const double2 d = *reinterpret_cast<const double2*>(p);
+        r.v[0] = d.x;
+        r.v[1] = d.y;
+    } else {
+        r.v[0] = *p;
     }
-    t[e * R + r] = alpha * D.tmul[e] * a;
+    return r;
 }
 
-// the same for two adjacent right-hand sides r, r + 1 (R even, r even): one
-// 16-byte load per partial (P and t are 16-byte aligned, (o R + r) is
-// even), each rhs summed in the order above; loads in batches of 8
-__device__ inline void psum_mm_item2(const DevPlan& D, const double* __restrict__ Pb, double* __restrict__ t, int R,
-                                     double alpha, int64_t g) {
-    const int h = R >> 1;
-    const int64_t e = g / h;
-    const int r = 2 * int(g % h);
-    const int32_t o = D.pt_off[e], k = D.tstride[e], cnt = D.pt_cnt[e];
-    const double2* src = reinterpret_cast<const double2*>(Pb + int64_t(o) * R + r);
-    const int64_t step = int64_t(k) * h;                 // in pairs
-    double ax = 0, ay = 0;
-    for (int32_t r0 = 0; r0 < cnt; r0 += kSegLen) {
-        const int32_t n = min(kSegLen, cnt - r0);
-        double sx = 0, sy = 0;
+template <int NP, int NI, int B>
+__device__ inline void psum_mm_items(const DevPlan& D, const double* __restrict__ Pb, double* __restrict__ t, int R,
+                                     double alpha, int64_t g0, int64_t stride, int64_t g1) {
+    const int h = R / NP;                                // items per entry
+    int64_t e[NI];
+    int r[NI];
+    int32_t o[NI], k[NI], cnt[NI];
 #pragma unroll
-        for (int b = 0; b < kSegLen; b += 8) {
-            if (b >= n) break;
-            double2 v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = (b + u < n) ? src[(int64_t(r0) + b + u) * step] : double2{0.0, 0.0};
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (b + u < n) { sx += v[u].x; sy += v[u].y; }
-        }
-        ax += sx;
-        ay += sy;
+    for (int j = 0; j < NI; ++j) {
+        const int64_t g = g0 + j * stride;
+        const bool ok = g < g1;
+        e[j] = ok ? g / h : 0;
+        r[j] = ok ? NP * int(g - e[j] * h) : 0;
+        o[j] = D.pt_off[e[j]];
+        k[j] = D.tstride[e[j]];
+        cnt[j] = ok ? D.pt_cnt[e[j]] : 0;
     }
-    const double f = alpha * D.tmul[e];
-    reinterpret_cast<double2*>(t + e * R + r)[0] = double2{f * ax, f * ay};
+    PsumVec<NP> first[NI];
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+        first[j] = (cnt[j] > 0) ? psum_load<NP>(Pb + int64_t(o[j]) * R + r[j]) : PsumVec<NP>{};
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+        if (g0 + j * stride >= g1) break;
+        const double* src = Pb + int64_t(o[j]) * R + r[j];
+        const int64_t step = int64_t(k[j]) * R;
+        double a[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) a[q] = 0;
+        for (int32_t r0 = 0; r0 < cnt[j]; r0 += kSegLen) {
+            const int32_t n = min(kSegLen, cnt[j] - r0);
+            double s[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) s[q] = 0;
+            int32_t b = 0;
+            if (r0 == 0) {
+#pragma unroll
+                for (int q = 0; q < NP; ++q) s[q] += first[j].v[q];
+                b = 1;
+            }
+            for (; b < n; b += B) {
+                PsumVec<NP> v[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    v[u] = (b + u < n) ? psum_load<NP>(src + (int64_t(r0) + b + u) * step) : PsumVec<NP>{};
+#pragma unroll
+                for (int u = 0; u < B; ++u)
+                    if (b + u < n) {
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) s[q] += v[u].v[q];
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < NP; ++q) a[q] += s[q];
+        }
+        const double f = alpha * D.tmul[e[j]];
+        if constexpr (NP == 2)
+            reinterpret_cast<double2*>(t + e[j] * R + r[j])[0] = double2{f * a[0], f * a[1]};
+        else
+            t[e[j] * R + r[j]] = f * a[0];
+    }
 }
+
+// The sums of one plan take one item per thread, partials in rounds of 8
+// (pairs) or of a whole segment (single right-hand sides).  The batch's sum
+// units take up to four items per thread, partials in rounds of 4: its
+// entries mostly have one or two partials, and with one item per thread each
+// thread waited on two dependent load round trips per item (64 matrices,
+// step 12.5 -> 12.1 ms).  In a plan of many right-hand sides, four items per
+// thread made the sums slower (16 right-hand sides, FP64 1.40 -> 1.43 ms,
+// mixed 1.06 -> 1.10 ms): fewer waves share the entries' longer tails.
+template <int NP>
+constexpr int kPsumRound = NP == 2 ? 8 : kSegLen;
+constexpr int kBsumItems = 4;
+constexpr int kBsumRound = 4;
+
 
 // ---------------------------------------------------------------------------
 // R pass: one workgroup per row panel, wave w owns rows 16w .. 16w + 15
@@ -306,19 +352,23 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
         const int64_t cb = int64_t(Ch.col0) + gb0 * W;
         const int lim = min(SC, ncol - gb0 * W);         // columns of this block that exist
         __syncthreads();                                 // the MFMAs of the block before are done with zs
-        // The other families, for an even number of right-hand sides: a
-        // thread stages two neighbouring right-hand sides of a column with
+        // The other families, and the FP64-only kernel with two tiles, for
+        // an even number of right-hand sides: a thread stages two
+        // neighbouring right-hand sides of a column with
         // one 16-byte load (t and XT rows are 16-byte aligned then), so a
         // block takes half the rounds: 8 values per round for the hardware
         // formats, 4 for the packed ones, whose decode paths leave fewer
         // registers.  One load from the selected row (x or t) keeps the
         // registers of the scalar rounds (32 right-hand sides: mixed 1.81
         // -> 1.70 ms, block floating point 2.18 -> 1.93 ms, adaptive
-        // floating point 2.30 -> 2.06 ms).
+        // floating point 2.30 -> 2.06 ms; FP64 1.95 -> 1.85 ms).  The
+        // FP64-only kernels with one or four tiles keep their scalar rounds:
+        // the pairs cost them a wave per SIMD (batch step 12.5 -> 12.7 ms).
+        constexpr bool kPairs = !kUnroll || NTM == 2;
         constexpr int RS2 = RS / 2;                      // threads per column
         constexpr int NI2 = (SC * RS2 + kThr - 1) / kThr;
         constexpr int ZB2 = NI2 < (FAM == kFamHw ? 8 : 4) ? NI2 : (FAM == kFamHw ? 8 : 4);
-        if (!kUnroll && (kThr % RS2) == 0 && (R % 2) == 0) {
+        if (kPairs && (kThr % RS2) == 0 && (R % 2) == 0) {
             const int c2 = tid / RS2, r2 = 2 * (tid % RS2);
             auto round2 = [&](int j0) {
                 int32_t src[ZB2];

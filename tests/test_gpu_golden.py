@@ -44,6 +44,13 @@ def test_exact(cuda, case, prec, force):
     Y = t(g["Y0"])
     p.apply(g["alpha"], N.OP_NORMAL, t(g["X"]), Y)
     assert np.array_equal(Y.cpu().numpy(), g["Y"])
+    # 32 right-hand sides (an even count: staged in pairs), the golden
+    # columns repeated in a mixed order; every product is exact, so each
+    # column must equal its golden result bit for bit
+    idx = np.arange(32) * 5 % len(g["X"])
+    Y = t(g["Y0"][idx])
+    p.apply(g["alpha"], N.OP_NORMAL, t(g["X"][idx]), Y)
+    assert np.array_equal(Y.cpu().numpy(), g["Y"][idx])
     y32 = t(g["y0"], torch.float32)
     p.apply(g["alpha"], N.OP_NORMAL, t(g["x"], torch.float32), y32)
     assert np.array_equal(y32.cpu().numpy().astype(np.float64), g["y"])

@@ -84,7 +84,7 @@ def test_error_vs_fp64(cuda, prec, eps):
 
 
 @pytest.mark.parametrize("prec", ["bfp", "afl"])
-@pytest.mark.parametrize("nrhs", [2, 16, 33, 64])
+@pytest.mark.parametrize("nrhs", [2, 16, 32, 33, 64])
 def test_multi_rhs(cuda, sphere16k, prec, nrhs):
     """Several right-hand sides on the matrix cores with all of the family's
     widths in one plan (the V pass splits a width into accumulator sets of

@@ -121,7 +121,7 @@ def test_multi_rhs(cuda, log1d):
         assert rel(Yt.cpu().numpy(), Yr) <= TOL_FP64
 
 
-@pytest.mark.parametrize("nrhs", [2, 16, 33, 64, 70])
+@pytest.mark.parametrize("nrhs", [2, 16, 18, 32, 33, 64, 70])
 @pytest.mark.parametrize("prec", ["fp64", "valr"])
 def test_multi_rhs_matrix_cores(cuda, sphere16k, prec, nrhs):
     """Forward products with several right-hand sides run on the matrix
