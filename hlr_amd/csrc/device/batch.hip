@@ -69,14 +69,17 @@ template <typename XT, int FAM>
 __global__ __launch_bounds__(kThreads) void k_bv1(const BatchMat* __restrict__ mats,
                                                   const BatchUnit* __restrict__ units) {
     extern __shared__ double lds[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // wave-uniform wave index: the pack's table is read with scalar loads
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
     if (wave >= w.count) return;
     const VPack* pk = M.D.vpacks + w.first + wave;
     const XT* x = static_cast<const XT*>(M.x);
     double* xs = lds + wave * kPackXs;
-    HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(M.D, pk, x, M.part, xs, lane))
+    vpack_stage_x(pk, x, xs, lane);
+    wave_sync();
+    HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(M.D, pk, M.part, xs, lane))
 }
 
 template <typename XT>

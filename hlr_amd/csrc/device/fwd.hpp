@@ -15,32 +15,112 @@ constexpr int64_t kWholeZBytes = 48 * 1024;   // largest panel z block staged wh
 // V pass: one wave processes one pack of consecutive segments (lane = group)
 // ---------------------------------------------------------------------------
 
-template <typename T, typename XT>
-__device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp, const XT* __restrict__ x,
-                                 double* __restrict__ Pb, void* xsraw, int lane) {
+// The x slices of a pack's segments (up to kUnitRows values each) in the
+// wave's LDS, as doubles for the FP64 format and as floats for the others
+// (the accumulation type of the pack's format); before the format switch,
+// so that a kernel holds one copy of it.  The slices lie an odd distance
+// apart: the lanes of different segments read different rows i of their
+// own slice in the same step, which then fall into different LDS banks.
+// Slot p of the slices belongs to the last segment t with xoff[t] <= p
+// (xoff[0] = 0) and holds a value when p - xoff[t] < m[t] (the slot after
+// each slice is a gap).
+//
+// The pack pointer is uniform across the wave, so the segment table is
+// read with scalar loads issued together, and all x loads of the pack are
+// issued before the first store to LDS.  Reading the table field by field
+// through per-lane loads had made the start of a pack a chain of dependent
+// round trips (m, then xoff and x0, then the x values, for every segment in
+// turn).
+template <typename XT>
+__device__ inline void vpack_stage_x(const VPack* __restrict__ pkp, const XT* __restrict__ x, void* xsraw,
+                                     int lane) {
+    const VPack& pk = *pkp;     // read through the pointer: the arrays stay out of scratch
+    const int nseg = pk.nseg;
+    int32_t sm[kPackSegs], sxo[kPackSegs], sx0[kPackSegs];
+#pragma unroll
+    for (int t = 0; t < kPackSegs; ++t) {
+        sm[t] = pk.m[t];
+        sxo[t] = pk.xoff[t];
+        sx0[t] = pk.x0[t];
+    }
+    // the slots up to the end of the last slice, 64 per load
+    int xend = sxo[0] + sm[0];
+#pragma unroll
+    for (int t = 1; t < kPackSegs; ++t)
+        if (t == nseg - 1) xend = sxo[t] + sm[t];
+    constexpr int kXq = (kPackXs + 63) / 64;
+    double xv[kXq];
+    int xp[kXq];
+#pragma unroll
+    for (int q = 0; q < kXq; ++q) {
+        xp[q] = -1;
+        xv[q] = 0;
+        if (64 * q >= xend) continue;           // uniform: the loads before stay in flight
+        const int p = lane + 64 * q;
+        int o = sxo[0], mm = sm[0], b = sx0[0];
+#pragma unroll
+        for (int t = 1; t < kPackSegs; ++t)
+            if (t < nseg && p >= sxo[t]) {
+                o = sxo[t];
+                mm = sm[t];
+                b = sx0[t];
+            }
+        // an unconditional load (index 0 for gap and spare slots), so that
+        // no branch ends the batch of loads
+        const bool on = p - o < mm;
+        xp[q] = on ? p : -1;
+        xv[q] = double(x[on ? b + (p - o) : 0]);
+    }
+    if (pk.fmt == F64) {
+        double* xs = static_cast<double*>(xsraw);
+#pragma unroll
+        for (int q = 0; q < kXq; ++q)
+            if (xp[q] >= 0) xs[xp[q]] = xv[q];
+    } else {
+        float* xs = static_cast<float*>(xsraw);
+#pragma unroll
+        for (int q = 0; q < kXq; ++q)
+            if (xp[q] >= 0) xs[xp[q]] = float(xv[q]);
+    }
+}
+
+// The stream of a pack whose x slices vpack_stage_x has staged: lane =
+// group of the pack, W column sums per lane.
+template <typename T>
+__device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp, double* __restrict__ Pb,
+                                 void* xsraw, int lane) {
     using E = Elem<T>;
     using A = typename E::Acc;
     constexpr int W = E::W;
+    static_assert(sizeof(A) == (E::F == F64 ? 8 : 4), "x slices are doubles for FP64 only");
     const VPack& pk = *pkp;     // read through the pointer: the arrays stay out of scratch
-    // x slices of the segments (up to kUnitRows values each), an odd
-    // distance apart: the lanes of different segments read different rows i
-    // of their own slice in the same step, which then fall into different
-    // LDS banks
-    A* xs = static_cast<A*>(xsraw);                 // kPackXs values: segment j's slice at xoff[j]
     const int nseg = pk.nseg;
-    for (int j = 0; j < nseg; ++j) {
-        const int mj = pk.m[j], o = pk.xoff[j], x0 = pk.x0[j];
-        for (int i = lane; i < mj; i += 64) xs[o + i] = A(x[x0 + i]);
-    }
-    wave_sync();
-    const int ngr = pk.gofs[nseg];                  // groups of the pack = active lanes
-    int j = 0;
-    while (j + 1 < nseg && lane >= pk.gofs[j + 1]) ++j;
-    const int g = lane - pk.gofs[j];
-    const int m = (lane < ngr) ? pk.m[j] : 0;       // lanes past the pack's groups load nothing
-    const int64_t ldb = int64_t(pk.ng[j]) * 16;                 // bytes per row of the segment
-    const bytep base = elem_addr<T>(P.arena[E::F], pk.data_off[j]) + int64_t(g) * 16;
-    const A* xj = xs + pk.xoff[j];
+    int32_t gofs[kPackSegs + 1];
+#pragma unroll
+    for (int t = 0; t <= kPackSegs; ++t) gofs[t] = pk.gofs[t];
+    int ngr = gofs[1];                              // groups of the pack = active lanes
+#pragma unroll
+    for (int t = 2; t <= kPackSegs; ++t)
+        if (t == nseg) ngr = gofs[t];
+    A* xs = static_cast<A*>(xsraw);                 // kPackXs values: segment j's slice at xoff[j]
+    // the lane's segment: the last t < nseg with gofs[t] <= lane
+    int gj = gofs[0], mj = pk.m[0], xoj = pk.xoff[0];
+    int32_t ngj = pk.ng[0];
+    int64_t dj = pk.data_off[0];
+#pragma unroll
+    for (int t = 1; t < kPackSegs; ++t)
+        if (t < nseg && lane >= gofs[t]) {
+            gj = gofs[t];
+            mj = pk.m[t];
+            xoj = pk.xoff[t];
+            ngj = pk.ng[t];
+            dj = pk.data_off[t];
+        }
+    const int g = lane - gj;
+    const int m = (lane < ngr) ? mj : 0;            // lanes past the pack's groups load nothing
+    const int64_t ldb = int64_t(ngj) * 16;          // bytes per row of the segment
+    const bytep base = elem_addr<T>(P.arena[E::F], dj) + int64_t(g) * 16;
+    const A* xj = xs + xoj;
     A acc[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) acc[w] = A(0);
@@ -75,10 +155,10 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     for (int k = 0; k < W; ++k) {
         const int c = lane + 64 * k;
         const int gr = c / W;
-        int64_t vb = pk.vc0[0] - pk.gofs[0] * W;
+        int64_t vb = pk.vc0[0] - gofs[0] * W;
 #pragma unroll
         for (int t = 1; t < kPackSegs; ++t)
-            if (t < nseg && gr >= pk.gofs[t]) vb = pk.vc0[t] - pk.gofs[t] * W;
+            if (t < nseg && gr >= gofs[t]) vb = pk.vc0[t] - gofs[t] * W;
         const bool on = c < ncol;
         vd[k] = on ? P.vdest[int64_t(vb) + c] : -1;
         vs[k] = (E::kScaled && on) ? P.vscale[int64_t(vb) + c] : 1.0f;

@@ -47,11 +47,15 @@ namespace {
 template <typename XT, int FAM>
 __global__ __launch_bounds__(kThreads, 4) void k_vfwd(DevPlan P, const XT* __restrict__ x, double* __restrict__ Pb) {
     __shared__ double xs[kWaves][kPackXs];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index as a wave-uniform value: the pack's table is then read
+    // with scalar loads
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t pi = int64_t(blockIdx.x) * kWaves + wave;
     if (pi >= P.nvpacks) return;
     const VPack* pk = P.vpacks + pi;
-    HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(P, pk, x, Pb, xs[wave], lane))
+    vpack_stage_x(pk, x, xs[wave], lane);
+    wave_sync();
+    HMVM_FMT_SWITCH_FAM(FAM, pk->fmt, vpack_fwd<T>(P, pk, Pb, xs[wave], lane))
 }
 
 // t_e = alpha * S_e * sum of the entry's partials, in one kernel: the lanes
