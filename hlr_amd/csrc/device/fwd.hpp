@@ -206,6 +206,99 @@ __device__ inline void chunk_stage(const DevPlan& P, const RChunk& C, double alp
     }
 }
 
+// The z values of all chunks of a row panel, staged in LDS before any
+// stream (the panel's z block fits the LDS): one index space over the
+// chunks' columns in panel order, U columns per thread per round, so that
+// the loads of all chunks are in flight together.  Staged chunk by chunk,
+// each chunk's round ended with a wait for its own loads, and a panel of
+// four chunks waited on eight dependent round trips before its stream.  The
+// values are chunk_stage's: alpha x or t (0 for padding), times the column
+// scale in the scaled formats, rounded to the chunk's accumulation type.
+template <typename XT>
+__device__ inline void panel_stage(const DevPlan& P, const RowPanel& R, double alpha, const XT* __restrict__ x,
+                                   const double* __restrict__ t, unsigned char* zdyn, int tid) {
+    int total = 0;
+    for (int ci = 0; ci < R.nchunks; ++ci) {
+        const RChunk& C = P.rchunks[R.chunk0 + ci];
+        total += C.ngroups * fmt_width(C.fmt);
+    }
+    const int64_t col0 = (R.nchunks > 0) ? P.rchunks[R.chunk0].col0 : 0;
+    constexpr int U = 4;
+    for (int j0 = 0; j0 < total; j0 += U * kThreads) {
+        // the chunk of each of the thread's columns (a scan of the panel's
+        // chunks, uniform across the workgroup)
+        int64_t col[U];
+        int zo[U], f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            col[u] = -1;
+            zo[u] = 0;
+            f[u] = F64;
+        }
+        int start = 0;
+        for (int ci = 0; ci < R.nchunks; ++ci) {
+            const RChunk& C = P.rchunks[R.chunk0 + ci];
+            const int n = C.ngroups * fmt_width(C.fmt);
+            if (start < j0 + U * kThreads && start + n > j0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int j = j0 + tid + u * kThreads - start;
+                    if (j >= 0 && j < n) {
+                        col[u] = C.col0 + j;
+                        f[u] = C.fmt;
+                        zo[u] = C.zoff + j * (C.fmt == F64 ? int(sizeof(double)) : int(sizeof(float)));
+                    }
+                }
+            }
+            start += n;
+        }
+        // unconditional loads at safe indices (the panel's first column,
+        // x[0], t[0]), masked afterwards: a load under a divergent
+        // condition ends its block with a wait.  zscale exists in plans
+        // with scaled formats, t when there are low-rank entries.
+        int64_t cl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) cl[u] = (col[u] >= 0) ? col[u] : col0;
+        int32_t s[U];
+        float sc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t v = P.colsrc[cl[u]];
+            s[u] = (col[u] >= 0) ? v : kPadCol;
+        }
+        if (P.zscale != nullptr) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float v = P.zscale[cl[u]];
+                sc[u] = fmt_scaled(f[u]) ? v : 1.0f;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) sc[u] = 1.0f;
+        }
+        double xv[U], tv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = double(x[s[u] >= 0 ? s[u] : 0]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) tv[u] = 0.0;
+        if (P.ntent > 0) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) tv[u] = t[(s[u] < 0 && s[u] != kPadCol) ? ~s[u] : 0];
+        }
+        double z[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) z[u] = (s[u] >= 0) ? alpha * xv[u] : (s[u] == kPadCol ? 0.0 : tv[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (col[u] < 0) continue;
+            if (f[u] == F64)
+                *reinterpret_cast<double*>(zdyn + zo[u]) = z[u];
+            else
+                *reinterpret_cast<float*>(zdyn + zo[u]) = float(fmt_scaled(f[u]) ? z[u] * double(sc[u]) : z[u]);
+        }
+    }
+}
+
 template <typename T>
 __device__ inline double chunk_range(const DevPlan& P, const RChunk& C, int m, const void* zbuf, int gb, int ge,
                                      int lane) {

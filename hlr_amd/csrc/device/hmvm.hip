@@ -107,11 +107,7 @@ __global__ __launch_bounds__(kThreads) void k_ufwd(DevPlan P, double alpha, cons
     const RowPanel R = P.rp[blockIdx.x];
     double acc = 0;
     if (kWhole) {
-        for (int ci = 0; ci < R.nchunks; ++ci) {
-            const RChunk C = P.rchunks[R.chunk0 + ci];
-            void* zb = zdyn + C.zoff;
-            HMVM_FMT_SWITCH_FAM(FAM, C.fmt, chunk_stage<T>(P, C, alpha, x, t, zb, tid))
-        }
+        panel_stage(P, R, alpha, x, t, zdyn, tid);
         __syncthreads();
         for (int ci = 0; ci < R.nchunks; ++ci) {
             const RChunk C = P.rchunks[R.chunk0 + ci];
