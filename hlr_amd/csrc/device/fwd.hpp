@@ -87,7 +87,7 @@ __device__ inline void vpack_stage_x(const VPack* __restrict__ pkp, const XT* __
 // The stream of a pack whose x slices vpack_stage_x has staged: lane =
 // group of the pack, W column sums per lane.
 template <typename T>
-__device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp, double* __restrict__ Pb,
+__device__ inline void vpack_fwd(const DevPlan& P, const VPack* pkp, double* __restrict__ Pb,
                                  void* xsraw, int lane) {
     using E = Elem<T>;
     using A = typename E::Acc;
@@ -145,28 +145,49 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* __restrict__ pkp
     for (int w = 0; w < W; ++w) res[lane * W + w] = acc[w];
     wave_sync();
     // Column c of the pack is column c - W gofs[s] of segment s: its table
-    // index is vc0[s] - W gofs[s] + c, selected in registers from the pack's
-    // (uniform) segment bounds, so that all W table loads of a lane are
-    // issued together (no load waits on another).
-    const int ncol = ngr * W;
-    int32_t vd[W];
-    float vs[W];
+    // index is vc0[s] - W gofs[s] + c.  The segment bases are read once
+    // (scalar loads) and selected per column in registers; then all 2 W
+    // table loads of a lane are issued, unconditionally (index 0 for
+    // columns past the pack), before the first store.  Selecting with
+    // per-column loads of vc0 under divergent conditions had made every
+    // selection wait for all loads in flight: up to W (1 + segments)
+    // dependent round trips per pack.
+    // The table is read again here, after the barrier (the pack pointer is
+    // not restrict-qualified, so the compiler does not merge these reads
+    // with the earlier ones across its fences), and none of it stays in
+    // scalar registers through the stream: at the kernel's SGPR limit that
+    // had pushed the decode's compares onto VCC, with hazard NOPs.  (V
+    // column indices are below 2^31: the plan builder checks.)
+    const int nsg = pk.nseg;
+    int32_t eg[kPackSegs + 1], vc[kPackSegs];
+#pragma unroll
+    for (int t = 0; t <= kPackSegs; ++t) eg[t] = pk.gofs[t];
+#pragma unroll
+    for (int t = 0; t < kPackSegs; ++t) vc[t] = int32_t(pk.vc0[t]) - eg[t] * W;
+    int ecol = eg[1] * W;
+#pragma unroll
+    for (int t = 2; t <= kPackSegs; ++t)
+        if (t == nsg) ecol = eg[t] * W;
+    int32_t ix[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int c = lane + 64 * k;
         const int gr = c / W;
-        int64_t vb = pk.vc0[0] - gofs[0] * W;
+        int32_t vb = vc[0];
 #pragma unroll
-        for (int t = 1; t < kPackSegs; ++t)
-            if (t < nseg && gr >= gofs[t]) vb = pk.vc0[t] - gofs[t] * W;
-        const bool on = c < ncol;
-        vd[k] = on ? P.vdest[int64_t(vb) + c] : -1;
-        vs[k] = (E::kScaled && on) ? P.vscale[int64_t(vb) + c] : 1.0f;
+        for (int t = 1; t < kPackSegs; ++t) vb = (t < nsg && gr >= eg[t]) ? vc[t] : vb;
+        ix[k] = (c < ecol) ? vb + c : 0;
     }
+    int32_t vd[W];
+    float vs[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) vd[k] = P.vdest[ix[k]];
+#pragma unroll
+    for (int k = 0; k < W; ++k) vs[k] = E::kScaled ? P.vscale[ix[k]] : 1.0f;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int c = lane + 64 * k;
-        if (vd[k] >= 0) Pb[vd[k]] = double(res[c]) * double(vs[k]);
+        if (c < ecol && vd[k] >= 0) Pb[vd[k]] = double(res[c]) * double(vs[k]);
     }
 }
 
