@@ -58,7 +58,8 @@ __global__ __launch_bounds__(256) void k_psum_mm(DevPlan D, const double* __rest
 }
 
 template <int NTM, typename YT, int FAM>
-__global__ __launch_bounds__(kThr) void k_ufwd_mm(DevPlan P, double alpha, const double* __restrict__ XTb,
+__global__ __launch_bounds__(kThr, (mm_min_waves<NTM, FAM>())) void k_ufwd_mm(DevPlan P, double alpha,
+                                                                          const double* __restrict__ XTb,
                                                   const double* __restrict__ t, YT* __restrict__ Y, int64_t ldy,
                                                   int R) {
     extern __shared__ double zbuf[];               // 64 x mm_stride<NTM>(), sized at launch

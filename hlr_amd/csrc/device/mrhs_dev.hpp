@@ -85,6 +85,14 @@ __device__ inline void xtrans_tile(const XT* __restrict__ X, int64_t ldx, int64_
 // one or two tiles takes a quarter or half of the LDS of four, and more of
 // them share a CU (32 right-hand sides, mixed plan: 2.05 -> 1.80 ms).
 template <int NTM> constexpr int mm_stride() { return 16 * NTM + 1; }
+// Waves per SIMD the R-pass kernel is compiled for: 5 for the FP64-only
+// kernel with one tile, whose unconditional staging loads took it from 88
+// to 92 VGPRs and from 5 to 4 waves (16 right-hand sides 1.40 -> 1.51 ms);
+// at 5 it fits in 96 without spilling.  Bounds that would restore the
+// four-tile kernels' occupancy, or the batch's one-tile kernel's, spill.
+template <int NTM, int FAM> constexpr int mm_min_waves() {
+    return (NTM == 1 && FAM == kFamF64) ? 5 : 1;
+}
 __host__ __device__ inline size_t mm_lds_bytes(int Rp) { return size_t(64) * (Rp + 1) * sizeof(double); }
 
 template <int NTM, typename T>
@@ -334,7 +342,10 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
     const int kk = lane >> 4, n = lane & 15;
     const bytep base = elem_addr<T>(P.arena[M::F], Ch.data_off) + int64_t(row < m ? row : 0) * 16;
     // The z values of a staged block (rhs fastest: contiguous reads of XT
-    // and t).  When RS divides kThr, a thread keeps one rhs index r and
+    // and t).  Every load of a round is unconditional, at a safe index
+    // (the block's first column, XT row 0) for padding, and masked
+    // afterwards: a load under a divergent condition ended its block with a
+    // wait, which had serialised the rounds load by load.  When RS divides kThr, a thread keeps one rhs index r and
     // stages columns c0 + j kThr / RS in rounds of ZB: a round issues its
     // column-source loads, then its value loads, so that a block waits for
     // a few load round trips rather than two per value.  Otherwise (RS = 48)
@@ -376,21 +387,36 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
 #pragma unroll
                 for (int u = 0; u < ZB2; ++u) {
                     const int c = c2 + (j0 + u) * (kThr / RS2);
-                    src[u] = (j0 + u < NI2 && c < lim) ? P.colsrc[cb + c] : kPadCol;
-                    if constexpr (E::kScaled) sc[u] = (j0 + u < NI2 && c < lim) ? P.zscale[cb + c] : 0.f;
+                    const bool ok = j0 + u < NI2 && c < lim;
+                    const int64_t ci = cb + (ok ? c : 0);
+                    const int32_t v = P.colsrc[ci];
+                    src[u] = ok ? v : kPadCol;
+                    if constexpr (E::kScaled) {
+                        const float w = P.zscale[ci];
+                        sc[u] = ok ? w : 0.f;
+                    }
+                }
+                // all value loads of the round, then all stores
+                double2 zv[ZB2];
+#pragma unroll
+                for (int u = 0; u < ZB2; ++u) {
+                    const int32_t s = src[u];
+                    const bool on = s != kPadCol && r2 < R;
+                    // one load from a selected row: x (times alpha) or t;
+                    // XT row 0 for padding
+                    const double* src2 =
+                        !on ? XTb : (s >= 0) ? XTb + int64_t(s) * R + r2 : t + int64_t(~s) * R + r2;
+                    zv[u] = *reinterpret_cast<const double2*>(src2);
                 }
 #pragma unroll
                 for (int u = 0; u < ZB2; ++u) {
                     const int c = c2 + (j0 + u) * (kThr / RS2);
                     const int32_t s = src[u];
-                    double2 z{0.0, 0.0};
-                    if (s != kPadCol && r2 < R) {
-                        // one load from a selected row: x (times alpha) or t
-                        const double* src2 = (s >= 0) ? XTb + int64_t(s) * R + r2 : t + int64_t(~s) * R + r2;
-                        z = *reinterpret_cast<const double2*>(src2);
-                        if (s >= 0) { z.x *= alpha; z.y *= alpha; }
-                        if constexpr (E::kScaled) { z.x *= double(sc[u]); z.y *= double(sc[u]); }
-                    }
+                    const bool on = s != kPadCol && r2 < R;
+                    double2 z = zv[u];
+                    if (s >= 0) { z.x *= alpha; z.y *= alpha; }
+                    if constexpr (E::kScaled) { z.x *= double(sc[u]); z.y *= double(sc[u]); }
+                    if (!on) z = double2{0.0, 0.0};
                     if (j0 + u < NI2 && c < SC) {
                         zs[c * ZS + r2] = A(z.x);
                         zs[c * ZS + r2 + 1] = A(z.y);
@@ -407,18 +433,33 @@ __device__ void rchunk_mm(const DevPlan& P, const RChunk& Ch, int m, double alph
 #pragma unroll
                 for (int u = 0; u < ZB; ++u) {
                     const int c = c0 + (j0 + u) * (kThr / RS);
-                    src[u] = (j0 + u < NI && c < lim) ? P.colsrc[cb + c] : kPadCol;
-                    if constexpr (E::kScaled) sc[u] = (j0 + u < NI && c < lim) ? P.zscale[cb + c] : 0.f;
+                    const bool ok = j0 + u < NI && c < lim;
+                    const int64_t ci = cb + (ok ? c : 0);
+                    const int32_t v = P.colsrc[ci];
+                    src[u] = ok ? v : kPadCol;
+                    if constexpr (E::kScaled) {
+                        const float w = P.zscale[ci];
+                        sc[u] = ok ? w : 0.f;
+                    }
+                }
+                // all value loads of the round, then all stores
+                double zv[ZB];
+#pragma unroll
+                for (int u = 0; u < ZB; ++u) {
+                    const int32_t s = src[u];
+                    const bool on = s != kPadCol && r0 < R;
+                    const double* p1 = !on ? XTb : (s >= 0) ? XTb + int64_t(s) * R + r0 : t + int64_t(~s) * R + r0;
+                    zv[u] = *p1;
                 }
 #pragma unroll
                 for (int u = 0; u < ZB; ++u) {
                     const int c = c0 + (j0 + u) * (kThr / RS);
                     const int32_t s = src[u];
-                    double z = 0;
-                    if (s != kPadCol && r0 < R) {
-                        z = (s >= 0) ? alpha * XTb[int64_t(s) * R + r0] : t[int64_t(~s) * R + r0];
-                        if constexpr (E::kScaled) z *= double(sc[u]);
-                    }
+                    const bool on = s != kPadCol && r0 < R;
+                    const double v = zv[u];
+                    double z = (s >= 0) ? alpha * v : v;
+                    if constexpr (E::kScaled) z *= double(sc[u]);
+                    if (!on) z = 0;
                     if (j0 + u < NI && c < SC) zs[c * ZS + r0] = A(z);
                 }
             };
