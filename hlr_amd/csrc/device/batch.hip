@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kThreads) void k_bvm(const BatchMat* __restrict__ m
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const BatchUnit w = units[blockIdx.x];
     const BatchMat& M = mats[w.mat];
-    vseg_mm_dispatch<NTM, FAM>(M.D, M.D.fitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
+    vseg_mm_dispatch<NTM, FAM, false>(M.D, M.D.fitems[w.first], M.xt, M.R, M.part, lds, tid, wave, lane);
 }
 
 // partial sums of one member: items (entry, pair of right-hand sides) when R

@@ -46,7 +46,10 @@ __global__ __launch_bounds__(kThr) void k_vfwd_mm(DevPlan P, const double* __res
     extern __shared__ double xs[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const VItem sg = P.fitems[blockIdx.x];
-    vseg_mm_dispatch<NTM, FAM>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
+    // the batched x-slice staging except for the hardware formats with four
+    // tiles, whose kernel it takes from 3 to 2 waves per SIMD (64
+    // right-hand sides, mixed plan 3.07 -> 3.23 ms)
+    vseg_mm_dispatch<NTM, FAM, !(NTM == 4 && FAM == kFamHw)>(P, sg, XTb, R, Pb, xs, tid, wave, lane);
 }
 
 // one item (entry, NP right-hand sides) per thread

@@ -95,7 +95,7 @@ template <int NTM, int FAM> constexpr int mm_min_waves() {
 }
 __host__ __device__ inline size_t mm_lds_bytes(int Rp) { return size_t(64) * (Rp + 1) * sizeof(double); }
 
-template <int NTM, typename T>
+template <int NTM, typename T, bool kBatchedStage>
 __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const double* __restrict__ XTb, int R,
                                   double* __restrict__ Pb, double* xsraw, int tid, int wave, int lane) {
     using M = Mm<T>;
@@ -110,12 +110,41 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
     constexpr int XS = mm_stride<NTM>();
     A* xs = reinterpret_cast<A*>(xsraw);                 // [64][XS]: row, rhs
     constexpr int RS = 16 * NTM;                         // staged rhs slots (constant divisor)
-    // rows i0 .. i0 + 63 of the segment's x slice
+    // rows i0 .. i0 + 63 of the segment's x slice (rhs fastest: contiguous
+    // reads of XT).  kBatchedStage: four values per thread at a time, their
+    // loads unconditional (XT entry 0 for padding, masked afterwards) and
+    // all issued before the LDS stores; loaded one at a time under a
+    // divergent condition, each value waited for the one before (64
+    // right-hand sides, FP64 plan 3.15 -> 2.96 ms).  The batch's V-pass
+    // kernel keeps the one-at-a-time form: there the batched form took the
+    // FP64 four-tile kernel from 104 to 248 VGPRs, with spills.
+    static_assert((64 * RS) % (4 * kThr) == 0, "whole rounds of four values per thread");
     auto stage = [&](int i0) {
         const int mb = min(64, m - i0);
-        for (int idx = tid; idx < 64 * RS; idx += kThr) {
-            const int i = idx / RS, r = idx - i * RS;     // rhs fastest: contiguous reads of XT
-            if (r < Rp) xs[i * XS + r] = (i < mb && r < R) ? A(XTb[(sg.x0 + i0 + i) * R + r]) : A(0);
+        if constexpr (kBatchedStage) {
+#pragma unroll 1
+            for (int q0 = 0; q0 < 64 * RS; q0 += 4 * kThr) {
+                A v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int idx = q0 + q * kThr + tid;
+                    const int i = idx / RS, r = idx - i * RS;
+                    const bool ok = i < mb && r < R;
+                    const A a = A(XTb[ok ? (sg.x0 + i0 + i) * R + r : 0]);
+                    v[q] = ok ? a : A(0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int idx = q0 + q * kThr + tid;
+                    const int i = idx / RS, r = idx - i * RS;
+                    if (r < Rp) xs[i * XS + r] = v[q];
+                }
+            }
+        } else {
+            for (int idx = tid; idx < 64 * RS; idx += kThr) {
+                const int i = idx / RS, r = idx - i * RS;
+                if (r < Rp) xs[i * XS + r] = (i < mb && r < R) ? A(XTb[(sg.x0 + i0 + i) * R + r]) : A(0);
+            }
         }
     };
     const bool single = m <= 64;                         // one block: staged once for every pass
@@ -193,10 +222,10 @@ __device__ inline void vseg_mm_wg(const DevPlan& P, const VItem& sg, const doubl
 
 // NTM: tiles of 16 right-hand sides the accumulators are sized for
 // (compile time); the call handles ceil(R / 16) <= NTM of them
-template <int NTM, int FAM = kFamAll>
+template <int NTM, int FAM = kFamAll, bool kBatchedStage = true>
 __device__ inline void vseg_mm_dispatch(const DevPlan& P, const VItem& sg, const double* __restrict__ XTb,
                                         int R, double* __restrict__ Pb, double* xs, int tid, int wave, int lane) {
-    HMVM_FMT_SWITCH_FAM(FAM, sg.fmt, vseg_mm_wg<NTM, T>(P, sg, XTb, R, Pb, xs, tid, wave, lane))
+    HMVM_FMT_SWITCH_FAM(FAM, sg.fmt, vseg_mm_wg<NTM, T, kBatchedStage>(P, sg, XTb, R, Pb, xs, tid, wave, lane))
 }
 
 // ---------------------------------------------------------------------------
