@@ -71,6 +71,9 @@ __global__ __launch_bounds__(256) void k_psum(const SumLane* __restrict__ lanes,
     if (g < nl) { ln = lanes[g]; il = lil[g]; }
     const int32_t e = ln.e;
     const int i = il & 255, L = il >> 8;
+    // the entry's factor, loaded with the partials (index 0 for idle lanes)
+    // rather than after the tree: one load round trip less
+    const double tm = tmul[e >= 0 ? e : 0];
     double v = 0;
     if (e >= 0) {
         const int32_t o = ln.off, c = ln.cnt, k = ln.stride;
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(256) void k_psum(const SumLane* __restrict__ lanes,
         const double w = __shfl_down(v, s, 64);
         if ((i & (2 * s - 1)) == 0 && i + s < L) v += w;
     }
-    if (e >= 0 && i == 0) t[e] = alpha * tmul[e] * v;
+    if (e >= 0 && i == 0) t[e] = alpha * tm * v;
 }
 
 // ---------------------------------------------------------------------------
@@ -224,13 +227,24 @@ __device__ inline void item_vtrn(const DevPlan& P, const ColPanel& C, const VIte
     // dot product over the lane's W columns is formed as in the forward
     // passes (FP32 for the FP32-accumulated formats), the row sums across
     // lanes in FP64
+    // All loads unconditional (lanes past the item's groups read group 0's
+    // entries and are masked afterwards): loaded under divergent
+    // conditions, each column's source and value had waited for the
+    // column before, two round trips per column.
     A sv[W];
+    int32_t ev[W];
+    float sc[W];
+    double xv[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) ev[w] = P.vsrc[it.vc0 + int64_t(g) * W + w];
+#pragma unroll
+    for (int w = 0; w < W; ++w) sc[w] = E::kScaled ? P.vscale[it.vc0 + int64_t(g) * W + w] : 1.0f;
+#pragma unroll
+    for (int w = 0; w < W; ++w) xv[w] = s[(on && ev[w] >= 0) ? ev[w] : 0];
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-        const int64_t vc = it.vc0 + int64_t(g) * W + w;
-        const int32_t e = on ? P.vsrc[vc] : -1;
-        double v = (e >= 0) ? s[e] : 0.0;
-        if (E::kScaled && on) v *= double(P.vscale[vc]);
+        double v = (on && ev[w] >= 0) ? xv[w] : 0.0;
+        if (E::kScaled && on) v *= double(sc[w]);
         sv[w] = A(v);
     }
     const int64_t ldb = int64_t(it.ng) * 16;            // bytes per row of the item
