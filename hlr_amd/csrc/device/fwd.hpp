@@ -126,8 +126,9 @@ __device__ inline void vpack_fwd(const DevPlan& P, const VPack* pkp, double* __r
     for (int w = 0; w < W; ++w) acc[w] = A(0);
     // rows per unrolled step (measured best; 8 for W >= 8 since the pack's
     // table and epilogue loads no longer wait on each other: AFL V pass
-    // 0.179 -> 0.163 ms, the others 0.5-1 % faster, where 4 had been best)
-    constexpr int kUnroll = (W >= 8) ? 8 : 16;
+    // 0.179 -> 0.163 ms, the others 0.5-1 % faster, where 4 had been best;
+    // 8 for FP32 as well: AFL V pass 0.164 -> 0.158 ms, others unchanged)
+    constexpr int kUnroll = (W >= 8 || W == 4) ? 8 : 16;
 #pragma unroll kUnroll
     for (int i = 0; i < m; ++i) {
         A v[W];
